@@ -1,0 +1,231 @@
+#!/usr/bin/env python3
+"""Benchmark of the explicit unstructured finite-volume Euler residual on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c3|c3mixed|c4|c5]
+
+A step is one residual evaluation plus the forward-Euler update (src/Base/Euler.cpp:85-97):
+pass A (gradient + limiter + CFL bound), the CFL reduction, pass B (flux + update +
+ghost BCs).  The state is resident in HBM before the timed region starts.  Prints ONE
+JSON line with cell-updates/s (interior cells x steps / s), the HBM roofline of the
+dominant kernel and the reference's CPU solver timed on this host on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import re
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+METRIC = "cell-updates/s per residual eval + % HBM peak; FP32-mixed vs FP64 at stated err"
+
+
+def alg_bytes(Nc: int, Nf: int, mixed: bool):
+    """Algorithmic HBM bytes of the minimal two-pass dataflow (cells incl. ghosts).
+    FP64:  pass A 104 Nc + 64 Nf (U, 1/V read; phi*G written; face geometry)
+           pass B 136 Nc + 64 Nf (U, phi*G, 1/V read; U written; face geometry)
+    mixed: pass A  68 Nc + 36 Nf;  pass B 100 Nc + 36 Nf."""
+    if mixed:
+        a, b = 68 * Nc + 36 * Nf, 100 * Nc + 36 * Nf
+    else:
+        a, b = 104 * Nc + 64 * Nf, 136 * Nc + 64 * Nf
+    return a, b, a + b
+
+
+def read_traffic(workload: str):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3
+    --pmc summary (profiles/pmc_<workload>.md, line 'traffic_bytes <kernel> <bytes>')."""
+    path = os.path.join(ROOT, "profiles", f"pmc_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    out = {}
+    for line in open(path):
+        m = re.match(r"\s*traffic_bytes\s+(\S+)\s+([0-9.eE+]+)", line)
+        if m:
+            out[m.group(1)] = float(m.group(2))
+    return out or None
+
+
+def cpu_baseline(case, ctx, dt: float, target_s: float):
+    """The reference's Base solver (oracle/_ref, built from its own sources) on one
+    core, on the same mesh in the GPU's RCM cell order, for a bounded number of steps."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
+    import oracle_lib as ol
+    from cfd_optimisations_amd.formats import renumber_cells, write_hmsh
+    hm = case.hmesh()
+    perm = ctx.permutation()
+    rm = renumber_cells(hm, perm)
+    nint = hm.Nc - (hm.Nf - hm.F_BC)
+    U0 = np.zeros((4, hm.Nc))
+    U0[:, perm] = ctx.get_state()
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "bench.hmsh")
+        write_hmsh(path, rm)
+        kind = "reference" if ol.ref_available(ol.REF_SO) else "port"
+        bcmap = [0] + [int(b) for b in case.bctypes]
+        if kind == "reference":
+            rc = ol.RefCase(path, case.gamma, bcmap, case.qbc, ol.REF_SO)
+            rc.zero_bc_iv()
+            run = lambda U, n: rc.run(U, n, dt)
+        else:
+            od = ol.OracleDomain(rm, case.gamma, case.bctypes, case.qbc)
+            od.zero_bc_iv()
+            run = lambda U, n: od.run(U, n, dt=dt)[0]
+        t0 = time.perf_counter()
+        U = run(U0, 1)
+        one = time.perf_counter() - t0
+        n = max(1, min(200, int(target_s / max(one, 1e-6))))
+        t0 = time.perf_counter()
+        run(U, n)
+        sec = time.perf_counter() - t0
+    cpu = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"value": nint * n / sec, "unit": "cell-updates/s", "cores": 1, "kind": kind,
+            "sample": f"{'reference Base (oracle/_ref)' if kind == 'reference' else 'oracle port'}, same mesh in "
+                      f"RCM order, {n} steps of fixed dt after 1 warm-up step, 1 thread on {cpu}",
+            "seconds": round(sec, 3)}
+
+
+def build_case(name: str):
+    from cfd_optimisations_amd import cases as C
+    from cfd_optimisations_amd import lib as L
+    W = C.WORKLOADS
+    if name == "c2":
+        w = W["c2_riemann_fp64"]
+        return C.riemann_case(w["nx"], w["ny"], seed=0), w, L.FP64, \
+            f"c2: four-quadrant Riemann problem on [0,1]^2, {w['nx']}x{w['ny']} quads -> {2 * w['nx'] * w['ny']:,} " \
+            f"triangles, nodes jittered +-20% (seed 0), cells shuffled then RCM-renumbered, FP64, CFL 0.5"
+    if name in ("c3", "c3mixed"):
+        w = W["c3_ramp_16m"]
+        prec = L.MIXED if name == "c3mixed" else L.FP64
+        return C.ramp_case(w["nx"], w["ny"], w["mach"], w["angle"], seed=0), w, prec, \
+            f"{name}: Mach {w['mach']} flow over a {w['angle']} deg ramp, {w['nx']}x{w['ny']} quads -> " \
+            f"{2 * w['nx'] * w['ny']:,} triangles, {'FP32-mixed' if prec else 'FP64'}, CFL 0.5"
+    raise SystemExit(f"unknown workload {name}")
+
+
+def run_single(name: str, steps: int, warmup: int, cpu_s: float, graph: bool, secondary: bool, tune: bool):
+    import numpy as np
+    from cfd_optimisations_amd import cases as C
+    from cfd_optimisations_amd import lib as L
+    case, w, prec, desc = build_case(name)
+    steps = steps or w["steps"]
+    cfl = w["cfl"]
+    flags = L.FLAG_GRAPH if graph else 0
+    t0 = time.perf_counter()
+    ctx = C.make_context(case, prec, flags)
+    setup_s = time.perf_counter() - t0
+    inf = ctx.info()
+    if tune:
+        ctx.autotune(steps=10, cfl=cfl, cache=os.path.join(tempfile.gettempdir(), "hcfd_tune.txt"))
+    ctx.step(warmup, cfl=cfl)
+    # timed region: exactly `steps` steps, synchronised on both sides
+    t0 = time.perf_counter()
+    tm = ctx.time_steps(steps, cfl=cfl)
+    wall = time.perf_counter() - t0
+    ms_step = 1e3 * wall / steps
+    value = inf.Nint * steps / wall
+    a_b, b_b, step_b = alg_bytes(inf.Nc, inf.Nf, prec == L.MIXED)
+    kern = {"pass_a": (tm["pass_a_ms"], a_b), "pass_b": (tm["pass_b_ms"], b_b)}
+    dom = max(kern, key=lambda k: kern[k][0])
+    dms, dbytes = kern[dom]
+    achieved = dbytes / (dms * 1e-3) / 1e9
+    traffic = read_traffic(name)
+    line = {
+        "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64" if prec == L.MIXED else "f64", "data": "synthetic",
+        "config": {"workload": desc, "Nc": inf.Nc, "Nint": inf.Nint, "Nf": inf.Nf, "steps": steps, "cfl": cfl,
+                   "schedule": ctx.schedule(), "graph": graph, "setup_s": round(setup_s, 2),
+                   "rcm_bandwidth": [inf.bandwidth_in, inf.bandwidth_out], "face_colours": inf.ncolours},
+        "hbm_pct_step": 100.0 * step_b / (ms_step * 1e-3) / (HBM_PEAK_GBS * 1e9),
+        "alg_bytes_per_step": step_b,
+        "kernel_ms": {"pass_a": tm["pass_a_ms"], "pass_b": tm["pass_b_ms"], "dt_reduce": tm["dt_reduce_ms"],
+                      "event_wall_per_step": tm["wall_ms"] / steps},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "alg_bytes": dbytes,
+                     "traffic": (traffic or {}).get(dom)},
+        "fault": ctx.fault(),
+    }
+    if cpu_s > 0:
+        dt = ctx.last_dt()
+        line["cpu_baseline"] = cpu_baseline(case, ctx, dt, cpu_s)
+        line["gpu_over_cpu"] = value / line["cpu_baseline"]["value"]
+    else:
+        line["cpu_baseline"] = None
+    ctx.close()
+    if secondary and name == "c2":
+        line["secondary"] = secondary_c3(steps=None)
+    return line
+
+
+def secondary_c3(steps=None):
+    """FP64 and FP32-mixed on the 16M-triangle ramp: throughput and the mixed path's
+    error per conserved variable against the FP64 GPU result after the same steps."""
+    import numpy as np
+    from cfd_optimisations_amd import cases as C
+    from cfd_optimisations_amd import lib as L
+    w = C.WORKLOADS["c3_ramp_16m"]
+    steps = steps or w["steps"]
+    case = C.ramp_case(w["nx"], w["ny"], w["mach"], w["angle"], seed=0)
+    out = []
+    states = {}
+    for prec in (L.FP64, L.MIXED):
+        ctx = C.make_context(case, prec, 0)
+        inf = ctx.info()
+        tm = ctx.time_steps(steps, cfl=w["cfl"])
+        _, _, step_b = alg_bytes(inf.Nc, inf.Nf, prec == L.MIXED)
+        ms = tm["wall_ms"] / steps
+        rec = {"workload": f"c3 ramp {w['nx']}x{w['ny']} quads, Mach {w['mach']}, {w['angle']} deg, "
+                           f"{'FP32-mixed' if prec else 'FP64'}, {steps} steps CFL {w['cfl']}",
+               "value": inf.Nint * steps / (tm["wall_ms"] * 1e-3), "unit": "cell-updates/s", "ms_per_step": ms,
+               "pass_a_ms": tm["pass_a_ms"], "pass_b_ms": tm["pass_b_ms"],
+               "hbm_pct_step": 100.0 * step_b / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), "fault": ctx.fault()}
+        states[prec] = ctx.get_state()
+        ctx.close()
+        out.append(rec)
+    Ua, Um = states[L.FP64], states[L.MIXED]
+    names = ("rho", "rho_u", "rho_v", "rho_E")
+    out[1]["error_vs_fp64"] = {
+        names[v]: {"max_abs": float(np.max(np.abs(Um[v] - Ua[v]))),
+                   "rel_l2": float(np.linalg.norm(Um[v] - Ua[v]) / np.linalg.norm(Ua[v]))} for v in range(4)}
+    out[1]["speedup_vs_fp64"] = out[1]["value"] / out[0]["value"]
+    return out
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=0, help="timed steps (default: the workload's)")
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c3mixed"])
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample length (0 = skip)")
+    ap.add_argument("--graph", action="store_true")
+    ap.add_argument("--tune", action="store_true", help="run the autotuner before timing")
+    ap.add_argument("--no-secondary", action="store_true")
+    a = ap.parse_args(argv)
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus}).",
+              file=sys.stderr)
+        return 2
+    line = run_single(a.workload, a.steps, a.warmup, a.cpu_seconds, a.graph, not a.no_secondary, a.tune)
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,182 @@
+"""Flow cases: meshes, boundary conditions and initial states.
+
+* the reference's own case (src/Test.cpp:51-74): wedge.hmsh, gamma 1.4, M = 2,
+  QIC = QBC = (1, 1, 0, 1/(gamma M^2)), BC map {-, SYM, SYM, SUPINLET,
+  SUPOUTLET} by colour tag, dt = 1e-3, 1000 steps;
+* a four-quadrant Riemann problem on [0,1]^2 (Lax-Liu configuration 3 states)
+  with supersonic outlets on all sides;
+* supersonic ramp / wedge channels: [0,L]x[0,H] with the lower wall ramping at
+  an angle from x_ramp, Mach M inflow (rho = 1, p = 1/gamma, u = M), slip walls
+  (Symmetry) top and bottom, SupInlet left, SupOutlet right;
+* seeded batches of wedges of uneven size, Mach number and angle.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import List, Optional, Tuple
+
+import numpy as np
+
+from . import lib as L
+
+GAMMA = 1.4
+
+
+@dataclass
+class Case:
+    mesh: object                 # lib.NativeMesh or formats.HMesh
+    bctypes: np.ndarray          # per BC ordinal
+    qbc: np.ndarray              # (NBC, 4) primitive BC states
+    qic: Optional[np.ndarray] = None      # uniform primitive initial state, or
+    Qcells: Optional[np.ndarray] = None   # (4, Nc) per-cell primitive initial state
+    gamma: float = GAMMA
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def Nc(self) -> int:
+        return self.mesh.view.Nc if hasattr(self.mesh, "view") else self.mesh.Nc
+
+    def n_interior(self) -> int:
+        hm = self.hmesh()
+        return hm.Nc - (hm.Nf - hm.F_BC)
+
+    def hmesh(self):
+        if not hasattr(self, "_hm"):
+            self._hm = self.mesh.to_hmesh() if hasattr(self.mesh, "to_hmesh") else self.mesh
+        return self._hm
+
+
+def bctypes_from_map(hm, bcmap_by_coltag) -> np.ndarray:
+    """Euler::gather_BCs: BC ordinal bc <- BCmap[coltag[Ncol-NBC+bc]] (src/Base/Euler.cpp:46-53)."""
+    return np.array([bcmap_by_coltag[hm.coltag[hm.Ncol - hm.NBCcol + b]] for b in range(hm.NBCcol)], np.int32)
+
+
+def reference_wedge_case(hm) -> Case:
+    """src/Test.cpp:51-64."""
+    M = 2.0
+    p = 1.0 / (GAMMA * M * M)
+    q = np.array([1.0, 1.0, 0.0, p])
+    bcmap = [L.SYMMETRY, L.SYMMETRY, L.SYMMETRY, L.SUPINLET, L.SUPOUTLET]
+    return Case(hm, bctypes_from_map(hm, bcmap), np.tile(q, (hm.NBCcol, 1)), qic=q,
+                meta={"dt": 1e-3, "steps": 1000, "name": "wedge.hmsh (src/Test.cpp)"})
+
+
+RIEMANN_STATES = {  # (rho, u, v, p), Lax-Liu configuration 3 states as quoted for this workload
+    "ne": (1.5, 0.0, 0.0, 1.5),
+    "nw": (0.5323, 1.206, 0.0, 0.3),
+    "sw": (0.138, 1.206, 1.206, 0.029),
+    "se": (0.5323, 0.0, 1.206, 0.3),
+}
+
+
+def cell_centroids(hm) -> np.ndarray:
+    nn = np.diff(hm.cn_off)
+    idx = np.repeat(np.arange(hm.Nc), nn)
+    xc = np.zeros((hm.Nc, 2))
+    np.add.at(xc, idx, hm.nodes[hm.cn_con])
+    return xc / nn[:, None]
+
+
+# Which of the four states sits in the NE, NW, SW, SE quadrant.  The textbook
+# placement ("ne","nw","sw","se") drives the reference method's conserved-variable
+# limiter to negative face pressures at the four-state junction from the first
+# step (the reference's own Base build goes non-finite within 20 steps); this
+# placement of the same four states runs the full 500 steps at 2000x1000.
+RIEMANN_PLACEMENT = ("ne", "sw", "se", "nw")
+
+
+def riemann_case(nx: int, ny: int, seed: int = 0, jitter: float = 0.2, shuffle: bool = True,
+                 placement=RIEMANN_PLACEMENT) -> Case:
+    mesh = L.NativeMesh.riemann(nx, ny, jitter, seed, shuffle)
+    hm = mesh.to_hmesh()
+    xc = cell_centroids(hm)
+    Q = np.zeros((4, hm.Nc))
+    quads = ((xc[:, 0] >= 0.5) & (xc[:, 1] >= 0.5), (xc[:, 0] < 0.5) & (xc[:, 1] >= 0.5),
+             (xc[:, 0] < 0.5) & (xc[:, 1] < 0.5), (xc[:, 0] >= 0.5) & (xc[:, 1] < 0.5))
+    for key, cond in zip(placement, quads):
+        Q[:, cond] = np.array(RIEMANN_STATES[key])[:, None]
+    bct = np.full(4, L.SUPOUTLET, np.int32)
+    qbc = np.tile(np.array(RIEMANN_STATES["ne"]), (4, 1))
+    c = Case(mesh, bct, qbc, Qcells=Q, meta={"name": f"riemann {nx}x{ny} seed {seed}", "nx": nx, "ny": ny})
+    c._hm = hm
+    return c
+
+
+def freestream(mach: float) -> np.ndarray:
+    return np.array([1.0, mach, 0.0, 1.0 / GAMMA])
+
+
+def ramp_case(nx: int, ny: int, mach: float, angle_deg: float, length: float = 1.0, height: float = 0.5,
+              x_ramp: float = 0.2, jitter: float = 0.2, seed: int = 0, shuffle: bool = False) -> Case:
+    mesh = L.NativeMesh.ramp(nx, ny, length, height, x_ramp, angle_deg, jitter, seed, shuffle)
+    q = freestream(mach)
+    # colours of the generator: lower wall, outlet, upper wall, inlet
+    bct = np.array([L.SYMMETRY, L.SUPOUTLET, L.SYMMETRY, L.SUPINLET], np.int32)
+    return Case(mesh, bct, np.tile(q, (4, 1)), qic=q,
+                meta={"name": f"ramp {nx}x{ny} M{mach:.3f} {angle_deg:.2f}deg", "nx": nx, "ny": ny,
+                      "mach": mach, "angle": angle_deg})
+
+
+class SplitMix64:
+    """Same generator as the library's mesh generators (Steele, Lea, Flood 2014)."""
+
+    def __init__(self, seed: int):
+        self.x = seed & 0xFFFFFFFFFFFFFFFF
+
+    def next(self) -> int:
+        self.x = (self.x + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+        z = self.x
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+        return z ^ (z >> 31)
+
+    def uniform(self) -> float:
+        return (self.next() >> 11) * (1.0 / 9007199254740992.0)
+
+
+def wedge_batch_specs(n: int, seed: int, mach: Tuple[float, float], angle: Tuple[float, float],
+                      tris: Tuple[float, float]) -> List[dict]:
+    """n wedges with Mach ~ U(mach), angle ~ U(angle) and triangle count ~
+    log-U(tris); grids of nx = 2 ny quads (aspect of the [0,1]x[0,0.5] domain)."""
+    r = SplitMix64(seed)
+    out = []
+    for k in range(n):
+        m = mach[0] + (mach[1] - mach[0]) * r.uniform()
+        a = angle[0] + (angle[1] - angle[0]) * r.uniform()
+        t = math.exp(math.log(tris[0]) + (math.log(tris[1]) - math.log(tris[0])) * r.uniform())
+        ny = max(2, int(round(math.sqrt(t / 4.0))))
+        out.append({"mach": m, "angle": a, "nx": 2 * ny, "ny": ny, "seed": seed * 100003 + k})
+    return out
+
+
+def wedge_batch(n: int, seed: int, mach=(1.5, 4.0), angle=(5.0, 25.0), tris=(20e3, 500e3)) -> List[Case]:
+    return [ramp_case(s["nx"], s["ny"], s["mach"], s["angle"], seed=s["seed"])
+            for s in wedge_batch_specs(n, seed, mach, angle, tris)]
+
+
+# Workloads as quoted for this residual (sizes in quads; each quad is 2 triangles).
+WORKLOADS = {
+    "c1_wedge_cpu": dict(kind="ramp", nx=224, ny=112, mach=2.5, angle=15.0, steps=2000, cfl=0.5),
+    "c2_riemann_fp64": dict(kind="riemann", nx=2000, ny=1000, steps=500, cfl=0.5, precision=L.FP64),
+    "c3_ramp_16m": dict(kind="ramp", nx=4000, ny=2000, mach=3.0, angle=20.0, steps=1000, cfl=0.5),
+    "c4_batch_fp64": dict(kind="batch", n=256, seed=1, mach=(1.5, 4.0), angle=(5.0, 25.0), tris=(20e3, 500e3),
+                          steps=2000, cfl=0.5, precision=L.FP64),
+    "c5_batch_mixed": dict(kind="batch", n=1024, seed=2, mach=(1.2, 5.0), angle=(5.0, 30.0), tris=(10e3, 1e6),
+                           steps=2000, cfl=0.5, precision=L.MIXED),
+}
+
+
+def make_context(case: Case, precision: int = L.FP64, flags: int = 0) -> "L.Context":
+    ctx = L.Context.create(case.mesh, precision, flags)
+    ctx.set_bcs(case.bctypes, case.qbc, case.gamma)
+    if case.Qcells is not None:
+        ctx.initialise_prims(case.Qcells)
+    else:
+        ctx.initialise(case.qic)
+    return ctx
+
+
+def make_batch(cases: List[Case], precision: int = L.FP64, flags: int = 0) -> "L.Context":
+    return L.Context.batch([c.mesh for c in cases], [(c.bctypes, c.qbc, c.qic) for c in cases], GAMMA,
+                           precision, flags)

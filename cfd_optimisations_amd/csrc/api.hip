@@ -1,0 +1,1031 @@
+// api.hip -- implementation of the C ABI declared in include/hcfd.h.
+//
+// A context (hcfd_ctx) is one or more mesh instances concatenated into a
+// single renumbered address space on one GPU, plus the state buffers of the
+// fused step.  See passes_impl.hpp for the method and mesh_build.hip for the
+// device-side mesh construction.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "mesh_build.hpp"
+#include "passes.hpp"
+#include "stages.hpp"
+
+namespace hcfd {
+
+static thread_local std::string g_last_error;
+
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+[[noreturn]] void fail(int code, const std::string& msg) { throw Error{code, msg}; }
+
+#define HIPCHK(x)                                                                                   \
+    do {                                                                                            \
+        hipError_t e_ = (x);                                                                        \
+        if (e_ != hipSuccess) fail(HCFD_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+static T* dmalloc(size_t n) {
+    T* p = nullptr;
+    hipError_t e = hipMalloc(&p, sizeof(T) * (n ? n : 1));
+    if (e != hipSuccess) fail(HCFD_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return p;
+}
+
+template <typename F>
+static int guarded(F&& f) {
+    try {
+        f();
+        return HCFD_OK;
+    } catch (const Error& e) {
+        set_last_error(e.msg);
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_last_error("host allocation failed");
+        return HCFD_ERR_NOMEM;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return HCFD_ERR_ARG;
+    }
+}
+
+}  // namespace hcfd
+
+using namespace hcfd;
+
+struct hcfd_ctx {
+    int precision = HCFD_FP64;
+    int flags = 0;
+    hipStream_t st = nullptr;
+    int device = 0;
+    int ncu = 256;
+    DeviceMesh m;
+    int Nc = 0, Nf = 0, Nint = 0, Nghost = 0, NcPad = 0, maxdeg = 0, deg_bucket = 3, ninst = 1;
+    double gamma = 1.4;
+    bool bcs_set = false;
+    // per instance, caller numbering
+    std::vector<int> inst_cell_off, inst_face_off, inst_bc_base, inst_nbc;
+    // device
+    BCEntry* bctab = nullptr;
+    int nbctab = 0;
+    int* d_old_of_new = nullptr;
+    int* d_ghost_inst = nullptr;
+    double* U[2] = {nullptr, nullptr};
+    int cur = 0;
+    double* G = nullptr;     // phi*G (fast) or G (exact), 8 per interior cell
+    double* PHI = nullptr;   // exact mode
+    float* G32 = nullptr;    // mixed
+    FaceRec32* faces32 = nullptr;
+    float* iv32 = nullptr;
+    double* pmin_first = nullptr;
+    double* pmin_last = nullptr;
+    unsigned long long* dtacc = nullptr;
+    double* dt = nullptr;
+    double* tdev = nullptr;
+    double* lastdt = nullptr;
+    int* fault = nullptr;
+    double* staging = nullptr;   // caller-order SoA scratch, 8 * Nc doubles
+    double* sU = nullptr;        // stage arrays (lazy)
+    double* sG = nullptr;
+    double* sPhi = nullptr;
+    double* sR = nullptr;
+    double* backup = nullptr;    // autotuner state backup
+    hcfd_schedule sched{256, 1, 1, 0, 0};
+    double t_host = 0.0;
+    double fixed_dt_loaded = NAN;
+    // captured two-step graph
+    hipGraphExec_t gexec = nullptr;
+    double graph_dt = NAN, graph_cfl = NAN;
+    int graph_cur = -1;
+    std::string graph_sched;
+
+    ~hcfd_ctx();
+};
+
+namespace {
+
+void free_ptr(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+std::string sched_key(const hcfd_schedule& s) {
+    char b[96];
+    snprintf(b, sizeof b, "%d/%d/%d/%d/%d", s.block, s.cells_per_thread, s.aosoa_width, s.persistent, s.xcd_remap);
+    return b;
+}
+
+void drop_graph(hcfd_ctx* c) {
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+    c->graph_cur = -1;
+}
+
+// ---- input validation (host) -------------------------------------------------
+
+void validate_view(const hcfd_mesh_view* v, int k) {
+    auto bad = [&](const std::string& w) { fail(HCFD_ERR_ARG, "mesh " + std::to_string(k) + ": " + w); };
+    if (!v) bad("null mesh view");
+    if (v->dim != 2) bad("only 2-D meshes are supported (dim=" + std::to_string(v->dim) + ")");
+    if (v->Nn <= 0 || v->Nc <= 0 || v->Nf <= 0 || v->Ncol <= 0 || v->NBCcol < 0 || v->NBCcol > v->Ncol)
+        bad("bad sizes");
+    if (!v->coloff || !v->nodes || !v->fn_con || !v->fc_con || !v->cn_off || !v->cn_con) bad("null array");
+    if (v->coloff[0] != 0 || v->coloff[v->Ncol] != v->Nf) bad("coloff must span [0, Nf]");
+    for (int c = 0; c < v->Ncol; c++)
+        if (v->coloff[c + 1] < v->coloff[c]) bad("coloff not monotone");
+    for (long long q = 0; q < 2LL * v->Nf; q++) {
+        if (v->fc_con[q] < 0 || v->fc_con[q] >= v->Nc) bad("fc_con index out of range");
+        if (v->fn_con[q] < 0 || v->fn_con[q] >= v->Nn) bad("fn_con index out of range");
+    }
+    if (v->cn_off[0] != 0) bad("cn_off[0] != 0");
+    for (int i = 0; i < v->Nc; i++)
+        if (v->cn_off[i + 1] < v->cn_off[i] + 1) bad("cell without nodes");
+    for (long long q = 0; q < v->cn_off[v->Nc]; q++)
+        if (v->cn_con[q] < 0 || v->cn_con[q] >= v->Nn) bad("cn_con index out of range");
+}
+
+// ---- creation ---------------------------------------------------------------------
+
+hcfd_ctx* create_ctx(int n, const hcfd_mesh_view* views, int precision, int flags) {
+    if (n <= 0) fail(HCFD_ERR_ARG, "no meshes");
+    if (precision != HCFD_FP64 && precision != HCFD_MIXED) fail(HCFD_ERR_ARG, "unknown precision");
+    if (precision == HCFD_MIXED && (flags & HCFD_FLAG_EXACT)) fail(HCFD_ERR_ARG, "EXACT applies to the FP64 path");
+    for (int k = 0; k < n; k++) validate_view(&views[k], k);
+
+    std::unique_ptr<hcfd_ctx> c(new hcfd_ctx());
+    c->precision = precision;
+    c->flags = flags;
+    c->ninst = n;
+    HIPCHK(hipGetDevice(&c->device));
+    HIPCHK(hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, c->device));
+    HIPCHK(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+
+    // concatenate instances
+    long long Nc = 0, Nf = 0, Nn = 0, ncn = 0;
+    for (int k = 0; k < n; k++) {
+        Nc += views[k].Nc;
+        Nf += views[k].Nf;
+        Nn += views[k].Nn;
+        ncn += views[k].cn_off[views[k].Nc];
+    }
+    if (Nc >= (1LL << 30) || Nf >= (1LL << 30) || ncn >= (1LL << 31)) fail(HCFD_ERR_ARG, "mesh too large");
+    std::vector<int> fc(2 * Nf), fn(2 * Nf), cn_off(Nc + 1), cn_con(ncn), bcid(Nf, -1);
+    std::vector<double> nodes(2 * Nn);
+    std::vector<unsigned char> ghost(Nc, 0);
+    c->inst_cell_off.assign(n + 1, 0);
+    c->inst_face_off.assign(n + 1, 0);
+    c->inst_bc_base.assign(n + 1, 0);
+    c->inst_nbc.assign(n, 0);
+    long long co = 0, fo = 0, no = 0, cno = 0;
+    int bcbase = 0;
+    for (int k = 0; k < n; k++) {
+        const hcfd_mesh_view& v = views[k];
+        const int F_BC = v.coloff[v.Ncol - v.NBCcol];
+        for (long long f = 0; f < v.Nf; f++) {
+            fc[2 * (fo + f)] = v.fc_con[2 * f] + (int)co;
+            fc[2 * (fo + f) + 1] = v.fc_con[2 * f + 1] + (int)co;
+            fn[2 * (fo + f)] = v.fn_con[2 * f] + (int)no;
+            fn[2 * (fo + f) + 1] = v.fn_con[2 * f + 1] + (int)no;
+        }
+        for (int col = v.Ncol - v.NBCcol; col < v.Ncol; col++)
+            for (int f = v.coloff[col]; f < v.coloff[col + 1]; f++) {
+                bcid[fo + f] = bcbase + (col - (v.Ncol - v.NBCcol));
+                ghost[co + v.fc_con[2 * f + 1]] = 1;
+            }
+        (void)F_BC;
+        for (long long q = 0; q < 2LL * v.Nn; q++) nodes[2 * no + q] = v.nodes[q];
+        for (int i = 0; i < v.Nc; i++) cn_off[co + i] = v.cn_off[i] + (int)cno;
+        for (long long q = 0; q < v.cn_off[v.Nc]; q++) cn_con[cno + q] = v.cn_con[q] + (int)no;
+        co += v.Nc;
+        fo += v.Nf;
+        no += v.Nn;
+        cno += v.cn_off[v.Nc];
+        c->inst_cell_off[k + 1] = (int)co;
+        c->inst_face_off[k + 1] = (int)fo;
+        c->inst_nbc[k] = v.NBCcol;
+        bcbase += v.NBCcol;
+        c->inst_bc_base[k + 1] = bcbase;
+    }
+    cn_off[Nc] = (int)cno;
+
+    BuildInput in;
+    in.Nc = (int)Nc;
+    in.Nf = (int)Nf;
+    in.Nn = (int)Nn;
+    in.ninst = n;
+    in.fc = fc.data();
+    in.fn = fn.data();
+    in.nodes = nodes.data();
+    in.cn_off = cn_off.data();
+    in.cn_con = cn_con.data();
+    in.bcid = bcid.data();
+    in.ghost = ghost.data();
+    in.inst_cell_off = c->inst_cell_off.data();
+    in.renumber = !(flags & HCFD_FLAG_NO_RENUMBER);
+    build_device_mesh(in, c->m, c->st);
+
+    c->Nc = c->m.Nc;
+    c->Nf = c->m.Nf;
+    c->Nint = c->m.Nint;
+    c->Nghost = c->m.Nghost;
+    c->NcPad = c->m.NcPad;
+    c->maxdeg = c->m.maxdeg;
+    c->deg_bucket = pass_degree_bucket(c->maxdeg);
+    if (c->deg_bucket < 0) fail(HCFD_ERR_MESH, "cells with more than 8 faces are not supported");
+    for (int k = 0; k < n; k++)
+        if (c->m.inst_int_off_h[k + 1] == c->m.inst_int_off_h[k]) fail(HCFD_ERR_MESH, "instance without interior cells");
+
+    // permutation + ghost instance on the device
+    c->d_old_of_new = dmalloc<int>(c->Nc);
+    HIPCHK(hipMemcpyAsync(c->d_old_of_new, c->m.old_of_new.data(), sizeof(int) * c->Nc, hipMemcpyHostToDevice, c->st));
+    {
+        std::vector<int> gi(std::max(c->Nghost, 1), 0);
+        for (int k = 0; k < n; k++)
+            for (int i = c->inst_cell_off[k]; i < c->inst_cell_off[k + 1]; i++)
+                if (ghost[i]) gi[c->m.new_of_old[i] - c->Nint] = k;
+        c->d_ghost_inst = dmalloc<int>(gi.size());
+        HIPCHK(hipMemcpyAsync(c->d_ghost_inst, gi.data(), sizeof(int) * gi.size(), hipMemcpyHostToDevice, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+    }
+
+    // state buffers (sized for the widest tile layout: NcPad is a multiple of 64)
+    const size_t P = c->NcPad;
+    c->U[0] = dmalloc<double>(4 * P);
+    c->U[1] = dmalloc<double>(4 * P);
+    HIPCHK(hipMemsetAsync(c->U[0], 0, sizeof(double) * 4 * P, c->st));
+    HIPCHK(hipMemsetAsync(c->U[1], 0, sizeof(double) * 4 * P, c->st));
+    if (precision == HCFD_FP64) {
+        c->G = dmalloc<double>(8 * P);
+        if (flags & HCFD_FLAG_EXACT) c->PHI = dmalloc<double>(4 * P);
+    } else {
+        c->G32 = dmalloc<float>(8 * P);
+        c->faces32 = dmalloc<FaceRec32>(c->Nf);
+        c->iv32 = dmalloc<float>(P);
+        // FP32 geometry from the FP64 records (same values rounded)
+        HIPCHK(launch_geom32(c->Nf, c->m.faces, c->faces32, (int)P, c->m.iv, c->iv32, c->st));
+    }
+    const int maxtiles = (c->Nint + WAVE - 1) / WAVE + 1;
+    c->pmin_first = dmalloc<double>(maxtiles);
+    c->pmin_last = dmalloc<double>(maxtiles);
+    c->dtacc = dmalloc<unsigned long long>(n);
+    c->dt = dmalloc<double>(n);
+    c->tdev = dmalloc<double>(n);
+    c->lastdt = dmalloc<double>(n);
+    c->fault = dmalloc<int>(n);
+    {
+        std::vector<unsigned long long> inf(n, (unsigned long long)0x7ff0000000000000ull);
+        HIPCHK(hipMemcpyAsync(c->dtacc, inf.data(), sizeof(unsigned long long) * n, hipMemcpyHostToDevice, c->st));
+        HIPCHK(hipMemsetAsync(c->tdev, 0, sizeof(double) * n, c->st));
+        HIPCHK(hipMemsetAsync(c->lastdt, 0, sizeof(double) * n, c->st));
+        HIPCHK(hipMemsetAsync(c->fault, 0, sizeof(int) * n, c->st));
+    }
+    c->staging = dmalloc<double>(8 * (size_t)c->Nc);
+    c->nbctab = std::max(bcbase, 1);
+    c->bctab = dmalloc<BCEntry>(c->nbctab);
+    HIPCHK(hipMemsetAsync(c->bctab, 0, sizeof(BCEntry) * c->nbctab, c->st));
+    c->sched.xcd_remap = 1;
+    HIPCHK(hipStreamSynchronize(c->st));
+    return c.release();
+}
+
+void fill_bctab(hcfd_ctx* c, int inst, int nbc, const int* types, const double* qbc, double gamma,
+                std::vector<BCEntry>& tab) {
+    if (nbc != c->inst_nbc[inst])
+        fail(HCFD_ERR_ARG, "instance " + std::to_string(inst) + " has " + std::to_string(c->inst_nbc[inst]) +
+                               " BC colours, got " + std::to_string(nbc));
+    for (int b = 0; b < nbc; b++) {
+        if (types[b] < HCFD_SYMMETRY || types[b] > HCFD_SUBOUTLET) fail(HCFD_ERR_ARG, "unknown BC type");
+        BCEntry e{};
+        e.type = types[b];
+        for (int v = 0; v < NV; v++) e.QBC[v] = qbc[NV * b + v];
+        cons_of_prims(gamma, e.QBC, e.UBC);
+        tab[c->inst_bc_base[inst] + b] = e;
+    }
+}
+
+void upload_bctab(hcfd_ctx* c, const std::vector<BCEntry>& tab) {
+    HIPCHK(hipMemcpyAsync(c->bctab, tab.data(), sizeof(BCEntry) * tab.size(), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    c->bcs_set = true;
+    drop_graph(c);
+}
+
+void require_bcs(hcfd_ctx* c) {
+    if (!c->bcs_set) fail(HCFD_ERR_STATE, "boundary conditions not set (hcfd_set_bcs)");
+}
+
+void apply_bc_cells(hcfd_ctx* c, double* U) {
+    HIPCHK(launch_bc_cells_state(c->sched.aosoa_width, c->Nint, c->Nc, c->NcPad, c->m.slots, c->m.faces, c->bctab,
+                                 c->m.ghost_bc, c->gamma, U, c->st));
+}
+
+void reset_time(hcfd_ctx* c) {
+    c->t_host = 0.0;
+    HIPCHK(hipMemsetAsync(c->tdev, 0, sizeof(double) * c->ninst, c->st));
+    HIPCHK(hipMemsetAsync(c->fault, 0, sizeof(int) * c->ninst, c->st));
+}
+
+// Caller-order SoA host arrays -> staging (pitch Nc).
+void upload_soa(hcfd_ctx* c, const double* const X[4], int ncomp_per_var) {
+    for (int v = 0; v < NV; v++) {
+        if (!X[v]) fail(HCFD_ERR_ARG, "null array");
+        HIPCHK(hipMemcpyAsync(c->staging + (size_t)v * ncomp_per_var * c->Nc, X[v],
+                              sizeof(double) * ncomp_per_var * c->Nc, hipMemcpyHostToDevice, c->st));
+    }
+}
+
+void download_soa(hcfd_ctx* c, double* const X[4], int ncomp_per_var) {
+    for (int v = 0; v < NV; v++) {
+        if (!X[v]) fail(HCFD_ERR_ARG, "null array");
+        HIPCHK(hipMemcpyAsync(X[v], c->staging + (size_t)v * ncomp_per_var * c->Nc,
+                              sizeof(double) * ncomp_per_var * c->Nc, hipMemcpyDeviceToHost, c->st));
+    }
+    HIPCHK(hipStreamSynchronize(c->st));
+}
+
+// ---- fused stepping -------------------------------------------------------------------
+
+struct Launch {
+    PassVariant v;
+    PassArgs a;
+    int grid = 0, block = 0;
+};
+
+Launch make_launch(hcfd_ctx* c, bool cfl, int cur) {
+    Launch L;
+    const hcfd_schedule& s = c->sched;
+    L.v.precision = c->precision;
+    L.v.exact = (c->flags & HCFD_FLAG_EXACT) != 0;
+    L.v.cfl = cfl;
+    L.v.width = s.aosoa_width;
+    L.v.maxdeg = c->deg_bucket;
+    PassArgs& a = L.a;
+    a.Nint = c->Nint;
+    a.Nc = c->Nc;
+    a.NcPad = c->NcPad;
+    a.ninst = c->ninst;
+    a.tile_cells = s.block;
+    a.ntiles = (c->Nint + a.tile_cells - 1) / a.tile_cells;
+    a.xcd_remap = s.xcd_remap;
+    a.tiles_per_xcd = (a.ntiles + NXCD - 1) / NXCD;
+    a.gamma = c->gamma;
+    a.slots = c->m.slots;
+    a.faces = c->m.faces;
+    a.faces32 = c->faces32;
+    a.iv = c->m.iv;
+    a.iv32 = c->iv32;
+    a.U = c->U[cur];
+    a.Unext = c->U[cur ^ 1];
+    a.G = c->G;
+    a.PHI = c->PHI;
+    a.G32 = c->G32;
+    a.inst_off = c->m.inst_int_off;
+    a.pmin_first = c->pmin_first;
+    a.pmin_last = c->pmin_last;
+    a.dtacc = c->dtacc;
+    a.dt = c->dt;
+    a.bctab = c->bctab;
+    a.ghost_bc = c->m.ghost_bc;
+    a.fault = c->fault;
+    L.block = s.block;
+    if (s.persistent > 0) {
+        int g = std::min(a.ntiles, s.persistent * c->ncu);
+        if (s.xcd_remap) g = std::max(NXCD, g / NXCD * NXCD);
+        L.grid = std::max(g, 1);
+    } else {
+        L.grid = s.xcd_remap ? NXCD * a.tiles_per_xcd : a.ntiles;
+    }
+    return L;
+}
+
+void load_fixed_dt(hcfd_ctx* c, double dt) {
+    if (c->fixed_dt_loaded == dt) return;
+    std::vector<double> h(c->ninst, dt);
+    HIPCHK(hipMemcpyAsync(c->dt, h.data(), sizeof(double) * c->ninst, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    c->fixed_dt_loaded = dt;
+}
+
+// Enqueues one step reading U[cur]; returns nothing, caller flips cur.
+void enqueue_step(hcfd_ctx* c, int cur, double cfl, hipEvent_t* ev = nullptr) {
+    const bool use_cfl = cfl > 0;
+    Launch L = make_launch(c, use_cfl, cur);
+    if (ev) HIPCHK(hipEventRecord(ev[0], c->st));
+    HIPCHK(launch_pass_a(L.v, L.a, L.grid, L.block, c->st));
+    if (ev) HIPCHK(hipEventRecord(ev[1], c->st));
+    if (use_cfl) HIPCHK(launch_dt_reduce(L.a, cfl, c->dt, c->tdev, c->lastdt, c->st));
+    if (ev) HIPCHK(hipEventRecord(ev[2], c->st));
+    HIPCHK(launch_pass_b(L.v, L.a, L.grid, L.block, c->st));
+    if (ev) HIPCHK(hipEventRecord(ev[3], c->st));
+}
+
+void ensure_graph(hcfd_ctx* c, double dt, double cfl) {
+    const std::string key = sched_key(c->sched);
+    if (c->gexec && c->graph_cur == c->cur && c->graph_dt == dt && c->graph_cfl == cfl && c->graph_sched == key)
+        return;
+    drop_graph(c);
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+    try {
+        enqueue_step(c, c->cur, cfl);
+        enqueue_step(c, c->cur ^ 1, cfl);
+    } catch (...) {
+        hipGraph_t junk = nullptr;
+        (void)hipStreamEndCapture(c->st, &junk);
+        if (junk) (void)hipGraphDestroy(junk);
+        throw;
+    }
+    HIPCHK(hipStreamEndCapture(c->st, &g));
+    hipError_t e = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIPCHK(e);
+    c->graph_cur = c->cur;
+    c->graph_dt = dt;
+    c->graph_cfl = cfl;
+    c->graph_sched = key;
+}
+
+void run_steps(hcfd_ctx* c, int nsteps, double dt, double cfl) {
+    require_bcs(c);
+    if (nsteps < 0) fail(HCFD_ERR_ARG, "nsteps < 0");
+    if (!(cfl > 0) && !(dt > 0)) fail(HCFD_ERR_ARG, "need dt > 0 or cfl > 0");
+    if (!(cfl > 0)) load_fixed_dt(c, dt);
+    int done = 0;
+    if ((c->flags & HCFD_FLAG_GRAPH) && nsteps >= 2) {
+        ensure_graph(c, dt, cfl);
+        for (; done + 2 <= nsteps; done += 2) HIPCHK(hipGraphLaunch(c->gexec, c->st));
+        // two steps leave U at the same parity
+    }
+    for (; done < nsteps; done++) {
+        enqueue_step(c, c->cur, cfl);
+        c->cur ^= 1;
+    }
+    if (!(cfl > 0))
+        for (int s = 0; s < nsteps; s++) c->t_host += dt;
+}
+
+double device_time(hcfd_ctx* c) {
+    double t = 0;
+    HIPCHK(hipMemcpyAsync(&t, c->tdev, sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return t;
+}
+
+void ensure_stage_arrays(hcfd_ctx* c) {
+    if (c->precision != HCFD_FP64) fail(HCFD_ERR_STATE, "stage entry points need an FP64 context");
+    if (c->sU) return;
+    const size_t P = c->NcPad;
+    c->sU = dmalloc<double>(4 * P);
+    c->sG = dmalloc<double>(8 * P);
+    c->sPhi = dmalloc<double>(4 * P);
+    c->sR = dmalloc<double>(4 * P);
+    HIPCHK(hipMemsetAsync(c->sU, 0, sizeof(double) * 4 * P, c->st));
+    HIPCHK(hipMemsetAsync(c->sG, 0, sizeof(double) * 8 * P, c->st));
+    HIPCHK(hipMemsetAsync(c->sPhi, 0, sizeof(double) * 4 * P, c->st));
+    HIPCHK(hipMemsetAsync(c->sR, 0, sizeof(double) * 4 * P, c->st));
+}
+
+StageArgs stage_args(hcfd_ctx* c) {
+    StageArgs a;
+    a.Nc = c->Nc;
+    a.Nint = c->Nint;
+    a.NcPad = c->NcPad;
+    a.maxdeg = c->maxdeg;
+    a.gamma = c->gamma;
+    a.slots = c->m.slots;
+    a.faces = c->m.faces;
+    a.iv = c->m.iv;
+    a.bctab = c->bctab;
+    a.ghost_bc = c->m.ghost_bc;
+    a.U = c->sU;
+    a.G = c->sG;
+    a.PHI = c->sPhi;
+    a.R = c->sR;
+    return a;
+}
+
+void run_stage(hcfd_ctx* c, int which, double dt) {
+    ensure_stage_arrays(c);
+    if (which == STAGE_BC_CELLS) require_bcs(c);
+    HIPCHK(stage_launch(which, stage_args(c), dt, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+}
+
+void change_width(hcfd_ctx* c, int neww) {
+    const int oldw = c->sched.aosoa_width;
+    if (oldw == neww) return;
+    // U -> caller-order staging -> new layout (identity permutation keeps internal order)
+    std::vector<int> ident(c->Nc);
+    for (int i = 0; i < c->Nc; i++) ident[i] = i;
+    int* d_ident = dmalloc<int>(c->Nc);
+    HIPCHK(hipMemcpyAsync(d_ident, ident.data(), sizeof(int) * c->Nc, hipMemcpyHostToDevice, c->st));
+    HIPCHK(launch_export(oldw, NV, c->Nc, d_ident, c->U[c->cur], c->staging, c->Nc, c->st));
+    HIPCHK(launch_import(neww, NV, c->Nc, d_ident, c->staging, c->Nc, c->U[c->cur], c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    (void)hipFree(d_ident);
+    c->sched.aosoa_width = neww;
+    drop_graph(c);
+}
+
+void check_schedule(const hcfd_ctx* c, const hcfd_schedule& s) {
+    if (s.block != 64 && s.block != 128 && s.block != 256) fail(HCFD_ERR_ARG, "block must be 64, 128 or 256");
+    if (s.cells_per_thread != 1) fail(HCFD_ERR_ARG, "cells_per_thread must be 1");
+    if (s.aosoa_width != 1 && s.aosoa_width != 8 && s.aosoa_width != 64)
+        fail(HCFD_ERR_ARG, "aosoa_width must be 1, 8 or 64");
+    if (s.persistent < 0 || s.persistent > 64) fail(HCFD_ERR_ARG, "persistent must be in [0, 64]");
+    (void)c;
+}
+
+}  // namespace
+
+hcfd_ctx::~hcfd_ctx() {
+    if (st) (void)hipStreamSynchronize(st);
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    free_device_mesh(m);
+    for (void* p : {(void*)bctab, (void*)d_old_of_new, (void*)d_ghost_inst, (void*)U[0], (void*)U[1], (void*)G,
+                    (void*)PHI, (void*)G32, (void*)faces32, (void*)iv32, (void*)pmin_first, (void*)pmin_last,
+                    (void*)dtacc, (void*)dt, (void*)tdev, (void*)lastdt, (void*)fault, (void*)staging, (void*)sU,
+                    (void*)sG, (void*)sPhi, (void*)sR, (void*)backup})
+        free_ptr(p);
+    if (st) (void)hipStreamDestroy(st);
+}
+
+// ============================================================================
+// C ABI
+// ============================================================================
+extern "C" {
+
+const char* hcfd_last_error(void) { return g_last_error.c_str(); }
+
+int hcfd_version(void) { return 10000; }
+
+int hcfd_device_count(int* n) {
+    return guarded([&] {
+        if (!n) fail(HCFD_ERR_ARG, "null");
+        HIPCHK(hipGetDeviceCount(n));
+    });
+}
+
+int hcfd_create(const hcfd_mesh_view* mesh, int precision, int flags, hcfd_ctx** out) {
+    return guarded([&] {
+        if (!out) fail(HCFD_ERR_ARG, "null out");
+        *out = create_ctx(1, mesh, precision, flags);
+    });
+}
+
+int hcfd_batch_create(int n, const hcfd_mesh_view* meshes, const hcfd_case* cases, double gamma, int precision,
+                      int flags, hcfd_ctx** out) {
+    return guarded([&] {
+        if (!out || !meshes || !cases) fail(HCFD_ERR_ARG, "null argument");
+        if (!(gamma > 1.0)) fail(HCFD_ERR_ARG, "gamma must exceed 1");
+        std::unique_ptr<hcfd_ctx> c(create_ctx(n, meshes, precision, flags));
+        c->gamma = gamma;
+        std::vector<BCEntry> tab(c->nbctab);
+        std::vector<double> uic(4 * (size_t)n);
+        for (int k = 0; k < n; k++) {
+            fill_bctab(c.get(), k, meshes[k].NBCcol, cases[k].bctype, cases[k].qbc, gamma, tab);
+            const double* q = cases[k].qic;
+            double u[NV];
+            u[0] = q[0];
+            for (int v = 1; v < NV; v++) u[v] = q[0] * q[v];
+            u[3] = q[3] / (gamma - 1) + 0.5 * q[0] * (q[1] * q[1] + q[2] * q[2]);
+            for (int v = 0; v < NV; v++) uic[4 * k + v] = u[v];
+        }
+        upload_bctab(c.get(), tab);
+        double* d_uic = dmalloc<double>(uic.size());
+        HIPCHK(hipMemcpyAsync(d_uic, uic.data(), sizeof(double) * uic.size(), hipMemcpyHostToDevice, c->st));
+        HIPCHK(launch_init_instances(c->sched.aosoa_width, c->Nint, c->Nc, c->m.inst_int_off, c->ninst,
+                                     c->d_ghost_inst, d_uic, c->U[c->cur], c->st));
+        apply_bc_cells(c.get(), c->U[c->cur]);
+        reset_time(c.get());
+        HIPCHK(hipStreamSynchronize(c->st));
+        (void)hipFree(d_uic);
+        *out = c.release();
+    });
+}
+
+int hcfd_destroy(hcfd_ctx* ctx) {
+    return guarded([&] { delete ctx; });
+}
+
+int hcfd_get_info(hcfd_ctx* c, hcfd_info* o) {
+    return guarded([&] {
+        if (!c || !o) fail(HCFD_ERR_ARG, "null");
+        o->Nc = c->Nc;
+        o->Nf = c->Nf;
+        o->Nint = c->Nint;
+        o->Nghost = c->Nghost;
+        o->ninst = c->ninst;
+        o->maxdeg = c->maxdeg;
+        o->precision = c->precision;
+        o->flags = c->flags;
+        o->bandwidth_in = c->m.bandwidth_in;
+        o->bandwidth_out = c->m.bandwidth_out;
+        o->ncolours = c->m.ncolours;
+        size_t P = c->NcPad;
+        long long b = 0;
+        b += (long long)c->maxdeg * P * sizeof(Slot) + (long long)c->Nf * sizeof(FaceRec64) + P * 8;
+        b += 8LL * P * 8;                         // U x2
+        b += (c->G ? 8LL * P * 8 : 0) + (c->PHI ? 4LL * P * 8 : 0) + (c->G32 ? 8LL * P * 4 : 0);
+        b += (c->faces32 ? (long long)c->Nf * sizeof(FaceRec32) + P * 4 : 0);
+        b += 8LL * c->Nc * 8 + 16LL * c->Nf + 16LL * c->Nc;  // staging, xf, xc
+        o->device_bytes = b;
+        o->sched = c->sched;
+    });
+}
+
+int hcfd_get_geometry(hcfd_ctx* c, double* xf, double* xc, double* rf, double* nf, double* Af, double* If,
+                      double* iv) {
+    return guarded([&] {
+        if (!c) fail(HCFD_ERR_ARG, "null");
+        if (xf) HIPCHK(hipMemcpy(xf, c->m.xf, sizeof(double) * 2 * c->Nf, hipMemcpyDeviceToHost));
+        if (xc) HIPCHK(hipMemcpy(xc, c->m.xc, sizeof(double) * 2 * c->Nc, hipMemcpyDeviceToHost));
+        if (rf || nf || Af || If) {
+            std::vector<FaceRec64> rec(c->Nf);
+            HIPCHK(hipMemcpy(rec.data(), c->m.faces, sizeof(FaceRec64) * c->Nf, hipMemcpyDeviceToHost));
+            for (int f = 0; f < c->Nf; f++) {
+                const FaceRec64& r = rec[c->m.face_new_of_old[f]];
+                if (rf) {
+                    rf[4 * f] = r.rL0; rf[4 * f + 1] = r.rL1; rf[4 * f + 2] = r.rR0; rf[4 * f + 3] = r.rR1;
+                }
+                if (nf) { nf[2 * f] = r.n0; nf[2 * f + 1] = r.n1; }
+                if (Af) Af[f] = r.A;
+                if (If) If[f] = r.If;
+            }
+        }
+        if (iv) {
+            std::vector<double> h(c->NcPad);
+            HIPCHK(hipMemcpy(h.data(), c->m.iv, sizeof(double) * c->NcPad, hipMemcpyDeviceToHost));
+            for (int i = 0; i < c->Nc; i++) iv[i] = h[c->m.new_of_old[i]];
+        }
+    });
+}
+
+int hcfd_get_cf(hcfd_ctx* c, int* cf_off, int* cf_con) {
+    return guarded([&] {
+        if (!c || !cf_off || !cf_con) fail(HCFD_ERR_ARG, "null");
+        std::memcpy(cf_off, c->m.cf_off.data(), sizeof(int) * c->m.cf_off.size());
+        std::memcpy(cf_con, c->m.cf_con.data(), sizeof(int) * c->m.cf_con.size());
+    });
+}
+
+int hcfd_get_permutation(hcfd_ctx* c, int* new_of_old) {
+    return guarded([&] {
+        if (!c || !new_of_old) fail(HCFD_ERR_ARG, "null");
+        std::memcpy(new_of_old, c->m.new_of_old.data(), sizeof(int) * c->Nc);
+    });
+}
+
+int hcfd_get_face_colours(hcfd_ctx* c, int* colour_of_face, int* ncolours) {
+    return guarded([&] {
+        if (!c) fail(HCFD_ERR_ARG, "null");
+        if (colour_of_face) std::memcpy(colour_of_face, c->m.colour.data(), sizeof(int) * c->Nf);
+        if (ncolours) *ncolours = c->m.ncolours;
+    });
+}
+
+int hcfd_set_bcs(hcfd_ctx* c, int nbc, const int* bctype, const double* qbc, double gamma) {
+    return guarded([&] {
+        if (!c || (nbc > 0 && (!bctype || !qbc))) fail(HCFD_ERR_ARG, "null argument");
+        if (c->ninst != 1) fail(HCFD_ERR_STATE, "batches take their BCs at creation");
+        if (!(gamma > 1.0)) fail(HCFD_ERR_ARG, "gamma must exceed 1");
+        c->gamma = gamma;
+        std::vector<BCEntry> tab(c->nbctab);
+        fill_bctab(c, 0, nbc, bctype, qbc, gamma, tab);
+        upload_bctab(c, tab);
+    });
+}
+
+int hcfd_initialise(hcfd_ctx* c, const double qic[4]) {
+    return guarded([&] {
+        if (!c || !qic) fail(HCFD_ERR_ARG, "null");
+        require_bcs(c);
+        // Euler::initialise, src/Base/Euler.cpp:62-77
+        double u[NV];
+        u[0] = qic[0];
+        for (int v = 1; v < NV; v++) u[v] = qic[0] * qic[v];
+        u[3] = qic[3] / (c->gamma - 1) + 0.5 * qic[0] * (qic[1] * qic[1] + qic[2] * qic[2]);
+        HIPCHK(hipMemcpyAsync(c->staging, u, sizeof(u), hipMemcpyHostToDevice, c->st));
+        HIPCHK(launch_init_uniform(c->sched.aosoa_width, c->Nc, c->staging, c->U[c->cur], c->st));
+        apply_bc_cells(c, c->U[c->cur]);
+        reset_time(c);
+        HIPCHK(hipStreamSynchronize(c->st));
+    });
+}
+
+int hcfd_initialise_prims(hcfd_ctx* c, const double* const Q[4]) {
+    return guarded([&] {
+        if (!c || !Q) fail(HCFD_ERR_ARG, "null");
+        require_bcs(c);
+        upload_soa(c, Q, 1);
+        HIPCHK(launch_init_prims(c->sched.aosoa_width, c->Nc, c->d_old_of_new, c->staging, c->Nc, c->gamma,
+                                 c->U[c->cur], c->st));
+        apply_bc_cells(c, c->U[c->cur]);
+        reset_time(c);
+        HIPCHK(hipStreamSynchronize(c->st));
+    });
+}
+
+int hcfd_set_state(hcfd_ctx* c, const double* const U[4]) {
+    return guarded([&] {
+        if (!c || !U) fail(HCFD_ERR_ARG, "null");
+        upload_soa(c, U, 1);
+        HIPCHK(launch_import(c->sched.aosoa_width, NV, c->Nc, c->d_old_of_new, c->staging, c->Nc, c->U[c->cur], c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+    });
+}
+
+int hcfd_get_state(hcfd_ctx* c, double* const U[4]) {
+    return guarded([&] {
+        if (!c || !U) fail(HCFD_ERR_ARG, "null");
+        HIPCHK(launch_export(c->sched.aosoa_width, NV, c->Nc, c->d_old_of_new, c->U[c->cur], c->staging, c->Nc, c->st));
+        download_soa(c, U, 1);
+    });
+}
+
+int hcfd_get_prims(hcfd_ctx* c, double* const Q[4]) {
+    return guarded([&] {
+        if (!c || !Q) fail(HCFD_ERR_ARG, "null");
+        HIPCHK(launch_prims(c->sched.aosoa_width, c->Nc, c->d_old_of_new, c->U[c->cur], c->gamma, c->staging, c->Nc,
+                            c->st));
+        download_soa(c, Q, 1);
+    });
+}
+
+int hcfd_stage_set(hcfd_ctx* c, int which, const double* const X[4]) {
+    return guarded([&] {
+        if (!c || !X) fail(HCFD_ERR_ARG, "null");
+        ensure_stage_arrays(c);
+        const int per = (which == HCFD_ARR_G) ? 2 : 1;
+        double* dst = which == HCFD_ARR_U ? c->sU : which == HCFD_ARR_G ? c->sG : which == HCFD_ARR_PHI ? c->sPhi
+                    : which == HCFD_ARR_R ? c->sR : nullptr;
+        if (!dst) fail(HCFD_ERR_ARG, "unknown array");
+        upload_soa(c, X, per);
+        HIPCHK(launch_soa_gather(c->Nc, NV * per, c->d_old_of_new, c->staging, c->Nc, dst, c->NcPad, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+    });
+}
+
+int hcfd_stage_get(hcfd_ctx* c, int which, double* const X[4]) {
+    return guarded([&] {
+        if (!c || !X) fail(HCFD_ERR_ARG, "null");
+        ensure_stage_arrays(c);
+        const int per = (which == HCFD_ARR_G) ? 2 : 1;
+        const double* src = which == HCFD_ARR_U ? c->sU : which == HCFD_ARR_G ? c->sG : which == HCFD_ARR_PHI ? c->sPhi
+                          : which == HCFD_ARR_R ? c->sR : nullptr;
+        if (!src) fail(HCFD_ERR_ARG, "unknown array");
+        HIPCHK(launch_soa_scatter(c->Nc, NV * per, c->d_old_of_new, src, c->NcPad, c->staging, c->Nc, c->st));
+        download_soa(c, X, per);
+    });
+}
+
+int hcfd_stage_gradient(hcfd_ctx* c) { return guarded([&] { run_stage(c, STAGE_GRADIENT, 0); }); }
+int hcfd_stage_limiter(hcfd_ctx* c) { return guarded([&] { run_stage(c, STAGE_LIMITER, 0); }); }
+int hcfd_stage_bc_grads(hcfd_ctx* c) { return guarded([&] { run_stage(c, STAGE_BC_GRADS, 0); }); }
+int hcfd_stage_bc_limiter(hcfd_ctx* c) { return guarded([&] { run_stage(c, STAGE_BC_LIMITER, 0); }); }
+int hcfd_stage_flux(hcfd_ctx* c) { return guarded([&] { run_stage(c, STAGE_FLUX, 0); }); }
+int hcfd_stage_update(hcfd_ctx* c, double dt) { return guarded([&] { run_stage(c, STAGE_UPDATE, dt); }); }
+int hcfd_stage_bc_cells(hcfd_ctx* c) { return guarded([&] { run_stage(c, STAGE_BC_CELLS, 0); }); }
+
+int hcfd_step(hcfd_ctx* c, int nsteps, double dt, double cfl, double* t_inout) {
+    return guarded([&] {
+        if (!c) fail(HCFD_ERR_ARG, "null");
+        run_steps(c, nsteps, dt, cfl);
+        if (cfl > 0) c->t_host = device_time(c);
+        else HIPCHK(hipStreamSynchronize(c->st));
+        if (t_inout) *t_inout = c->t_host;
+    });
+}
+
+int hcfd_get_last_dt(hcfd_ctx* c, double* dt) {
+    return guarded([&] {
+        if (!c || !dt) fail(HCFD_ERR_ARG, "null");
+        HIPCHK(hipMemcpyAsync(dt, c->lastdt, sizeof(double), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+    });
+}
+
+int hcfd_get_fault(hcfd_ctx* c, int* fault) {
+    return guarded([&] {
+        if (!c || !fault) fail(HCFD_ERR_ARG, "null");
+        std::vector<int> f(c->ninst);
+        HIPCHK(hipMemcpyAsync(f.data(), c->fault, sizeof(int) * c->ninst, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        *fault = 0;
+        for (int x : f) *fault |= x;
+    });
+}
+
+int hcfd_time_steps(hcfd_ctx* c, int nsteps, double dt, double cfl, float* ms_out) {
+    return guarded([&] {
+        if (!c || !ms_out || nsteps <= 0) fail(HCFD_ERR_ARG, "bad argument");
+        require_bcs(c);
+        if (!(cfl > 0)) load_fixed_dt(c, dt);
+        // One timed region: nsteps eager steps with an event pair around every
+        // launch; wall = first to last event.
+        std::vector<hipEvent_t> ev(4 * nsteps);
+        for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+        HIPCHK(hipStreamSynchronize(c->st));
+        for (int s = 0; s < nsteps; s++) {
+            enqueue_step(c, c->cur, cfl, &ev[4 * s]);
+            c->cur ^= 1;
+        }
+        if (!(cfl > 0))
+            for (int s = 0; s < nsteps; s++) c->t_host += dt;
+        HIPCHK(hipEventSynchronize(ev[4 * nsteps - 1]));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, ev[0], ev[4 * nsteps - 1]));
+        ms_out[0] = ms;
+        double sa = 0, sr = 0, sb = 0;
+        for (int s = 0; s < nsteps; s++) {
+            float a = 0, r = 0, b = 0;
+            HIPCHK(hipEventElapsedTime(&a, ev[4 * s], ev[4 * s + 1]));
+            HIPCHK(hipEventElapsedTime(&r, ev[4 * s + 1], ev[4 * s + 2]));
+            HIPCHK(hipEventElapsedTime(&b, ev[4 * s + 2], ev[4 * s + 3]));
+            sa += a;
+            sr += r;
+            sb += b;
+        }
+        ms_out[1] = (float)(sa / nsteps);
+        ms_out[2] = (float)(sb / nsteps);
+        ms_out[3] = (cfl > 0) ? (float)(sr / nsteps) : 0.0f;
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        if (cfl > 0) c->t_host = device_time(c);
+    });
+}
+
+int hcfd_set_schedule(hcfd_ctx* c, const hcfd_schedule* s) {
+    return guarded([&] {
+        if (!c) fail(HCFD_ERR_ARG, "null");
+        hcfd_schedule n = s ? *s : hcfd_schedule{256, 1, 1, 0, 1};
+        check_schedule(c, n);
+        change_width(c, n.aosoa_width);
+        c->sched = n;
+        drop_graph(c);
+    });
+}
+
+int hcfd_get_schedule(hcfd_ctx* c, hcfd_schedule* s) {
+    return guarded([&] {
+        if (!c || !s) fail(HCFD_ERR_ARG, "null");
+        *s = c->sched;
+    });
+}
+
+int hcfd_autotune(hcfd_ctx* c, int steps, double cfl, const char* cache_path, hcfd_schedule* best) {
+    return guarded([&] {
+        if (!c) fail(HCFD_ERR_ARG, "null");
+        require_bcs(c);
+        if (steps <= 0) steps = 10;
+        char key[160];
+        snprintf(key, sizeof key, "gfx950 Nc=%d Nf=%d Nint=%d ninst=%d prec=%d flags=%d deg=%d", c->Nc, c->Nf, c->Nint,
+                 c->ninst, c->precision, c->flags & HCFD_FLAG_EXACT, c->deg_bucket);
+        // cache lookup
+        if (cache_path) {
+            std::ifstream in(cache_path);
+            std::string line;
+            while (std::getline(in, line)) {
+                auto bar = line.find('|');
+                if (bar == std::string::npos || line.substr(0, bar) != key) continue;
+                hcfd_schedule s{};
+                if (sscanf(line.c_str() + bar + 1, "%d %d %d %d %d", &s.block, &s.cells_per_thread, &s.aosoa_width,
+                           &s.persistent, &s.xcd_remap) == 5) {
+                    check_schedule(c, s);
+                    change_width(c, s.aosoa_width);
+                    c->sched = s;
+                    drop_graph(c);
+                    if (best) *best = s;
+                    return;
+                }
+            }
+        }
+        const size_t P = c->NcPad;
+        if (!c->backup) c->backup = dmalloc<double>(4 * P);
+        const double dt_probe = 1e-12;  // only used when cfl <= 0: a negligible step keeps the state valid
+        std::vector<hcfd_schedule> cands;
+        for (int w : {1, 8, 64})
+            for (int b : {64, 128, 256})
+                for (int pers : {0, 4, 8})
+                    for (int x : {0, 1}) cands.push_back(hcfd_schedule{b, 1, w, pers, x});
+        const double t_save = c->t_host;
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        float best_ms = 1e30f;
+        hcfd_schedule bs = c->sched;
+        for (const auto& s : cands) {
+            change_width(c, s.aosoa_width);
+            c->sched = s;
+            drop_graph(c);
+            HIPCHK(hipMemcpyAsync(c->backup, c->U[c->cur], sizeof(double) * 4 * P, hipMemcpyDeviceToDevice, c->st));
+            const int cur0 = c->cur;
+            run_steps(c, 2, dt_probe, cfl);  // warm-up
+            HIPCHK(hipEventRecord(e0, c->st));
+            run_steps(c, steps, dt_probe, cfl);
+            HIPCHK(hipEventRecord(e1, c->st));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            c->cur = cur0;
+            HIPCHK(hipMemcpyAsync(c->U[c->cur], c->backup, sizeof(double) * 4 * P, hipMemcpyDeviceToDevice, c->st));
+            HIPCHK(hipStreamSynchronize(c->st));
+            if (ms < best_ms) {
+                best_ms = ms;
+                bs = s;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        change_width(c, bs.aosoa_width);
+        c->sched = bs;
+        drop_graph(c);
+        c->t_host = t_save;
+        // dt/time bookkeeping of the probes is discarded
+        HIPCHK(hipMemsetAsync(c->tdev, 0, sizeof(double) * c->ninst, c->st));
+        HIPCHK(hipMemsetAsync(c->fault, 0, sizeof(int) * c->ninst, c->st));
+        c->fixed_dt_loaded = NAN;
+        HIPCHK(hipStreamSynchronize(c->st));
+        if (t_save != 0.0) {
+            std::vector<double> tt(c->ninst, t_save);
+            HIPCHK(hipMemcpy(c->tdev, tt.data(), sizeof(double) * c->ninst, hipMemcpyHostToDevice));
+        }
+        if (cache_path) {
+            std::ofstream out(cache_path, std::ios::app);
+            out << key << "|" << bs.block << " " << bs.cells_per_thread << " " << bs.aosoa_width << " "
+                << bs.persistent << " " << bs.xcd_remap << "  # " << best_ms / steps << " ms/step\n";
+        }
+        if (best) *best = bs;
+    });
+}
+
+int hcfd_batch_get_state(hcfd_ctx* c, int inst, double* const U[4]) {
+    return guarded([&] {
+        if (!c || !U || inst < 0 || inst >= c->ninst) fail(HCFD_ERR_ARG, "bad argument");
+        HIPCHK(launch_export(c->sched.aosoa_width, NV, c->Nc, c->d_old_of_new, c->U[c->cur], c->staging, c->Nc, c->st));
+        const int o = c->inst_cell_off[inst], n = c->inst_cell_off[inst + 1] - o;
+        for (int v = 0; v < NV; v++)
+            HIPCHK(hipMemcpyAsync(U[v], c->staging + (size_t)v * c->Nc + o, sizeof(double) * n, hipMemcpyDeviceToHost,
+                                  c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+    });
+}
+
+int hcfd_batch_get_prims(hcfd_ctx* c, int inst, double* const Q[4]) {
+    return guarded([&] {
+        if (!c || !Q || inst < 0 || inst >= c->ninst) fail(HCFD_ERR_ARG, "bad argument");
+        HIPCHK(launch_prims(c->sched.aosoa_width, c->Nc, c->d_old_of_new, c->U[c->cur], c->gamma, c->staging, c->Nc,
+                            c->st));
+        const int o = c->inst_cell_off[inst], n = c->inst_cell_off[inst + 1] - o;
+        for (int v = 0; v < NV; v++)
+            HIPCHK(hipMemcpyAsync(Q[v], c->staging + (size_t)v * c->Nc + o, sizeof(double) * n, hipMemcpyDeviceToHost,
+                                  c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+    });
+}
+
+int hcfd_batch_set_state(hcfd_ctx* c, int inst, const double* const U[4]) {
+    return guarded([&] {
+        if (!c || !U || inst < 0 || inst >= c->ninst) fail(HCFD_ERR_ARG, "bad argument");
+        HIPCHK(launch_export(c->sched.aosoa_width, NV, c->Nc, c->d_old_of_new, c->U[c->cur], c->staging, c->Nc, c->st));
+        const int o = c->inst_cell_off[inst], n = c->inst_cell_off[inst + 1] - o;
+        for (int v = 0; v < NV; v++)
+            HIPCHK(hipMemcpyAsync(c->staging + (size_t)v * c->Nc + o, U[v], sizeof(double) * n, hipMemcpyHostToDevice,
+                                  c->st));
+        HIPCHK(launch_import(c->sched.aosoa_width, NV, c->Nc, c->d_old_of_new, c->staging, c->Nc, c->U[c->cur], c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+    });
+}
+
+int hcfd_batch_get_status(hcfd_ctx* c, double* t, double* last_dt, int* fault) {
+    return guarded([&] {
+        if (!c) fail(HCFD_ERR_ARG, "null");
+        const int n = c->ninst;
+        if (t) {
+            HIPCHK(hipMemcpyAsync(t, c->tdev, sizeof(double) * n, hipMemcpyDeviceToHost, c->st));
+        }
+        if (last_dt) HIPCHK(hipMemcpyAsync(last_dt, c->lastdt, sizeof(double) * n, hipMemcpyDeviceToHost, c->st));
+        if (fault) HIPCHK(hipMemcpyAsync(fault, c->fault, sizeof(int) * n, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+    });
+}
+
+int hcfd_batch_instance_size(hcfd_ctx* c, int inst, int* Nc, int* Nf) {
+    return guarded([&] {
+        if (!c || inst < 0 || inst >= c->ninst) fail(HCFD_ERR_ARG, "bad argument");
+        if (Nc) *Nc = c->inst_cell_off[inst + 1] - c->inst_cell_off[inst];
+        if (Nf) *Nf = c->inst_face_off[inst + 1] - c->inst_face_off[inst];
+    });
+}
+
+}  // extern "C"

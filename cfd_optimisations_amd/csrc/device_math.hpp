@@ -1,0 +1,149 @@
+// device_math.hpp -- per-face arithmetic of the residual, for gfx950 kernels.
+//
+// The FP64 functions keep the reference's expression order term by term so
+// that, compiled with -ffp-contract=off, each value is bit-identical to
+// Base's: EulerFlux (src/Base/Flux.cpp:89-97), Rusanov (:52-81) and the MUSCL
+// face state (:83-87).  The FP32 variants serve the mixed path.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+
+namespace hcfd {
+
+template <typename T>
+__device__ __forceinline__ T dmin(T a, T b) { return fmin(a, b); }
+template <typename T>
+__device__ __forceinline__ T dmax(T a, T b) { return fmax(a, b); }
+__device__ __forceinline__ float dmin(float a, float b) { return fminf(a, b); }
+__device__ __forceinline__ float dmax(float a, float b) { return fmaxf(a, b); }
+
+// Flux::EulerFlux, src/Base/Flux.cpp:89-97.
+template <typename T>
+__device__ __forceinline__ void euler_flux(T gma, const T U[NV], T n0, T n1, T F[NV]) {
+    const T gmo = gma - T(1);
+    const T ir = T(1) / U[0];
+    const T P = gmo * (U[3] - T(0.5) * ir * (U[1] * U[1] + U[2] * U[2]));
+    F[0] = n0 * U[1] + n1 * U[2];
+    F[1] = n0 * (U[1] * U[1] * ir + P) + n1 * U[2] * U[1] * ir;
+    F[2] = n0 * U[1] * U[2] * ir + n1 * (U[2] * U[2] * ir + P);
+    F[3] = n0 * (U[3] + P) * U[1] * ir + n1 * (U[3] + P) * U[2] * ir;
+}
+
+// Flux::Rusanov, src/Base/Flux.cpp:52-81.
+template <typename T>
+__device__ __forceinline__ void rusanov(T gma, const T UL[NV], const T UR[NV], T n0, T n1, T F[NV]) {
+    const T gmo = gma - T(1);
+    const T irl = T(1) / UL[0];
+    const T irr = T(1) / UR[0];
+    const T nvl = (n0 * UL[1] + n1 * UL[2]) * irl;
+    const T nvr = (n0 * UR[1] + n1 * UR[2]) * irr;
+    const T qql = T(0.5) * (UL[1] * UL[1] + UL[2] * UL[2]) * irl;
+    const T qqr = T(0.5) * (UR[1] * UR[1] + UR[2] * UR[2]) * irr;
+    const T pl = gmo * (UL[3] - qql);
+    const T pr = gmo * (UR[3] - qqr);
+    const T al = sqrt(gma * pl * irl);
+    const T ar = sqrt(gma * pr * irr);
+    const T S1 = dmax(fabs(nvl - al), fabs(nvr - ar));
+    const T S2 = dmax(fabs(nvl + al), fabs(nvr + ar));
+    const T S = dmax(S1, S2);
+    T FL[NV], FR[NV];
+    euler_flux(gma, UL, n0, n1, FL);
+    euler_flux(gma, UR, n0, n1, FR);
+#pragma unroll
+    for (int v = 0; v < NV; v++) F[v] = T(0.5) * ((FL[v] + FR[v]) - S * (UR[v] - UL[v]));
+}
+
+// Limiter face candidate, src/Base/Limiter.cpp:262-266 (eps = 1e-9, :20).
+template <typename T>
+__device__ __forceinline__ T limiter_face(T d, T umin_minus_u, T umax_minus_u) {
+    const T eps = T(1e-9);
+    T p = T(1);
+    p = (d > eps) ? dmin(T(1), umax_minus_u / d) : p;
+    p = (d < -eps) ? dmin(T(1), umin_minus_u / d) : p;
+    return p;
+}
+
+// Cell-state access with an AoSoA tile of width W: component c of cell i of an
+// array with NC components lives at (i/W)*NC*W + c*W + i%W.  W == 1 is AoS,
+// W >= the cell count behaves as SoA.
+template <int W, int NC>
+__device__ __forceinline__ long long cidx(int i, int c) {
+    return (long long)(i / W) * (NC * W) + c * W + (i % W);
+}
+
+template <int W, int NC, typename T>
+__device__ __forceinline__ void load_cell(const T* __restrict__ a, int i, T out[NC]) {
+    if constexpr (W == 1 && sizeof(T) == 8 && NC % 2 == 0) {
+        const double2* p = reinterpret_cast<const double2*>(a + (long long)i * NC);
+#pragma unroll
+        for (int c = 0; c < NC / 2; c++) {
+            double2 v = p[c];
+            out[2 * c] = v.x;
+            out[2 * c + 1] = v.y;
+        }
+    } else if constexpr (W == 1 && sizeof(T) == 4 && NC % 4 == 0) {
+        const float4* p = reinterpret_cast<const float4*>(a + (long long)i * NC);
+#pragma unroll
+        for (int c = 0; c < NC / 4; c++) {
+            float4 v = p[c];
+            out[4 * c] = v.x;
+            out[4 * c + 1] = v.y;
+            out[4 * c + 2] = v.z;
+            out[4 * c + 3] = v.w;
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < NC; c++) out[c] = a[cidx<W, NC>(i, c)];
+    }
+}
+
+template <int W, int NC, typename T>
+__device__ __forceinline__ void store_cell(T* __restrict__ a, int i, const T in[NC]) {
+    if constexpr (W == 1 && sizeof(T) == 8 && NC % 2 == 0) {
+        double2* p = reinterpret_cast<double2*>(a + (long long)i * NC);
+#pragma unroll
+        for (int c = 0; c < NC / 2; c++) p[c] = make_double2(in[2 * c], in[2 * c + 1]);
+    } else if constexpr (W == 1 && sizeof(T) == 4 && NC % 4 == 0) {
+        float4* p = reinterpret_cast<float4*>(a + (long long)i * NC);
+#pragma unroll
+        for (int c = 0; c < NC / 4; c++) p[c] = make_float4(in[4 * c], in[4 * c + 1], in[4 * c + 2], in[4 * c + 3]);
+    } else {
+#pragma unroll
+        for (int c = 0; c < NC; c++) a[cidx<W, NC>(i, c)] = in[c];
+    }
+}
+
+// Wave-level minimum over 64 lanes.
+__device__ __forceinline__ double wave_min(double x) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x = fmin(x, __shfl_xor(x, off, 64));
+    return x;
+}
+
+// Block-level minimum; every thread receives the result.  `sh` holds one
+// double per wave of the block.
+__device__ __forceinline__ double block_min(double x, double* sh) {
+    x = wave_min(x);
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int w = threadIdx.x / WAVE;
+    const int nw = (blockDim.x + WAVE - 1) / WAVE;
+    __syncthreads();
+    if (lane == 0) sh[w] = x;
+    __syncthreads();
+    double r = sh[0];
+    for (int k = 1; k < nw; k++) r = fmin(r, sh[k]);
+    return r;
+}
+
+// Interior-cell range of instance b is [off[b], off[b+1]); returns b for cell i.
+__device__ __forceinline__ int instance_of(const int* __restrict__ off, int ninst, int i) {
+    int lo = 0, hi = ninst - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= i) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+}  // namespace hcfd

@@ -1,0 +1,58 @@
+// passes.hpp -- the two fused passes of one explicit step and their launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+
+namespace hcfd {
+
+// Arguments of both passes (by value; every pointer is device memory).
+struct PassArgs {
+    int Nint = 0;          // interior cells [0, Nint); ghosts [Nint, Nc)
+    int Nc = 0;
+    int NcPad = 0;         // slot table pitch
+    int ninst = 1;
+    int ntiles = 0;        // ceil(Nint / cells_per_tile)
+    int tile_cells = 0;    // blockDim.x * cells per thread
+    int tiles_per_xcd = 0; // ceil(ntiles / 8) when xcd_remap
+    int xcd_remap = 0;
+    double gamma = 1.4;
+    const Slot* slots = nullptr;
+    const FaceRec64* faces = nullptr;
+    const FaceRec32* faces32 = nullptr;
+    const double* iv = nullptr;
+    const float* iv32 = nullptr;
+    const double* U = nullptr;   // current state (interior + ghosts)
+    double* Unext = nullptr;     // next state
+    double* G = nullptr;         // pass A output: limited gradient phi*G (or G in exact mode), 8 per cell
+    double* PHI = nullptr;       // exact mode: phi, 4 per cell
+    float* G32 = nullptr;        // mixed path: phi*G in FP32
+    const int* inst_off = nullptr;      // [ninst+1] interior ranges
+    double* pmin_first = nullptr;       // [ntiles] per-tile CFL partial of the tile's first instance
+    double* pmin_last = nullptr;        // [ntiles] ... of its last instance
+    unsigned long long* dtacc = nullptr;// [ninst] CFL minimum (bit pattern) of cells in neither
+    const double* dt = nullptr;         // [ninst] dt of this step
+    const BCEntry* bctab = nullptr;
+    const int* ghost_bc = nullptr;
+    int* fault = nullptr;               // [ninst]
+};
+
+// Variant selectors.
+struct PassVariant {
+    int precision = HCFD_FP64;  // HCFD_FP64 / HCFD_MIXED
+    bool exact = false;         // FP64 only: keep G and phi apart (bit-identical to Base)
+    bool cfl = false;           // pass A computes the CFL partials
+    int width = 1;              // AoSoA width: 1, 8, 64
+    int maxdeg = 3;             // slot count: 3, 4 or 8
+};
+
+// Launch the passes with `grid` x `block` threads on `st`.
+hipError_t launch_pass_a(const PassVariant& v, const PassArgs& a, int grid, int block, hipStream_t st);
+hipError_t launch_pass_b(const PassVariant& v, const PassArgs& a, int grid, int block, hipStream_t st);
+// Per-instance CFL reduction: dt[b] = cfl * min(...); t[b] += dt[b]; last[b] = dt[b].
+hipError_t launch_dt_reduce(const PassArgs& a, double cfl, double* dt, double* t, double* last, hipStream_t st);
+// Width / degree supported by the compiled variants.
+bool pass_variant_supported(const PassVariant& v);
+int pass_degree_bucket(int maxdeg);
+
+}  // namespace hcfd

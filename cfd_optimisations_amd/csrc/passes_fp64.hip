@@ -1,0 +1,95 @@
+// passes_fp64.hip -- FP64 variants of the fused passes, the CFL reduction and
+// the variant dispatch (see passes_impl.hpp for the method).
+#include "dispatch.hpp"
+#include "passes_impl.hpp"
+
+namespace hcfd {
+
+hipError_t launch_pass_a_mixed(const PassVariant& v, const PassArgs& a, int grid, int block, hipStream_t st);
+hipError_t launch_pass_b_mixed(const PassVariant& v, const PassArgs& a, int grid, int block, hipStream_t st);
+
+int pass_degree_bucket(int maxdeg) {
+    if (maxdeg <= 3) return 3;
+    if (maxdeg <= 4) return 4;
+    if (maxdeg <= 8) return 8;
+    return -1;
+}
+
+bool pass_variant_supported(const PassVariant& v) {
+    if (v.width != 1 && v.width != 8 && v.width != 64) return false;
+    if (v.maxdeg != 3 && v.maxdeg != 4 && v.maxdeg != 8) return false;
+    if (v.precision == HCFD_MIXED && v.exact) return false;
+    return v.precision == HCFD_FP64 || v.precision == HCFD_MIXED;
+}
+
+hipError_t launch_pass_a(const PassVariant& v, const PassArgs& a, int grid, int block, hipStream_t st) {
+    if (!pass_variant_supported(v) || block > 256 || block % WAVE) return hipErrorInvalidValue;
+    if (v.precision == HCFD_MIXED) return launch_pass_a_mixed(v, a, grid, block, st);
+    return with_width(v.width, [&](auto W) {
+        return with_degree(v.maxdeg, [&](auto MD) {
+            return with_bool(v.exact, [&](auto EX) {
+                return with_bool(v.cfl, [&](auto CF) {
+                    return with_bool(a.ninst > 1, [&](auto MU) {
+                        hipLaunchKernelGGL((k_pass_a_fp64<decltype(W)::value, decltype(MD)::value, decltype(EX)::value,
+                                                          decltype(CF)::value, decltype(MU)::value>),
+                                           dim3(grid), dim3(block), 0, st, a);
+                        return hipGetLastError();
+                    });
+                });
+            });
+        });
+    });
+}
+
+hipError_t launch_pass_b(const PassVariant& v, const PassArgs& a, int grid, int block, hipStream_t st) {
+    if (!pass_variant_supported(v) || block > 256 || block % WAVE) return hipErrorInvalidValue;
+    if (v.precision == HCFD_MIXED) return launch_pass_b_mixed(v, a, grid, block, st);
+    return with_width(v.width, [&](auto W) {
+        return with_degree(v.maxdeg, [&](auto MD) {
+            return with_bool(v.exact, [&](auto EX) {
+                return with_bool(a.ninst > 1, [&](auto MU) {
+                    hipLaunchKernelGGL((k_pass_b_fp64<decltype(W)::value, decltype(MD)::value, decltype(EX)::value,
+                                                      decltype(MU)::value>),
+                                       dim3(grid), dim3(block), 0, st, a);
+                    return hipGetLastError();
+                });
+            });
+        });
+    });
+}
+
+// Per-instance CFL reduction over the tile partials of pass A (block b <-> instance b).
+__global__ __launch_bounds__(1024) void k_dt_reduce(PassArgs a, double cfl, double* __restrict__ dt,
+                                                    double* __restrict__ t, double* __restrict__ last) {
+    __shared__ double sh[1024 / WAVE];
+    const int b = blockIdx.x;
+    const int c0 = a.inst_off[b], c1 = a.inst_off[b + 1];
+    double m = INFINITY;
+    if (c1 > c0) {
+        const int t0 = c0 / a.tile_cells, t1 = (c1 - 1) / a.tile_cells;
+        for (int tile = t0 + (int)threadIdx.x; tile <= t1; tile += blockDim.x) {
+            const int f0 = tile * a.tile_cells;
+            const int l0 = min(f0 + a.tile_cells, a.Nint) - 1;
+            const int bf = a.ninst > 1 ? instance_of(a.inst_off, a.ninst, f0) : 0;
+            const int bl = a.ninst > 1 ? instance_of(a.inst_off, a.ninst, l0) : 0;
+            if (bf == b) m = fmin(m, a.pmin_first[tile]);
+            else if (bl == b) m = fmin(m, a.pmin_last[tile]);
+        }
+    }
+    m = block_min(m, sh);
+    if (threadIdx.x == 0) {
+        m = fmin(m, __longlong_as_double((long long)a.dtacc[b]));
+        const double h = cfl * m;
+        dt[b] = h;
+        t[b] += h;
+        last[b] = h;
+        a.dtacc[b] = dbits(INFINITY);
+    }
+}
+
+hipError_t launch_dt_reduce(const PassArgs& a, double cfl, double* dt, double* t, double* last, hipStream_t st) {
+    hipLaunchKernelGGL(k_dt_reduce, dim3(a.ninst), dim3(1024), 0, st, a, cfl, dt, t, last);
+    return hipGetLastError();
+}
+
+}  // namespace hcfd

@@ -1,0 +1,212 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+FP64 with HCFD_FLAG_EXACT must be bit-identical to the oracle (which is itself
+bit-identical to the reference's Base sources built with -ffp-contract=off);
+the default FP64 path (limited gradient stored as phi*G) must stay within
+1e-12 normwise relative per variable; the mixed path within its stated bound.
+"""
+import numpy as np
+import pytest
+
+from cfd_optimisations_amd import cases as C
+from cfd_optimisations_amd import lib as L
+from cfd_optimisations_amd.formats import read_hsln
+
+import oracle_lib as ol
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12   # north-star FP64 bound (normwise relative, per variable)
+
+
+def relerr(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return np.array([np.linalg.norm(a[v] - b[v]) / max(np.linalg.norm(b[v]), 1e-300) for v in range(a.shape[0])])
+
+
+def oracle_for(case):
+    hm = case.hmesh()
+    od = ol.OracleDomain(hm, case.gamma, case.bctypes, case.qbc)
+    od.zero_bc_iv()
+    U0 = od.initialise_prims(case.Qcells) if case.Qcells is not None else od.initialise(case.qic)
+    return od, U0
+
+
+def test_device_present():
+    assert L.device_count() >= 1
+
+
+def test_wedge_geometry_csr_bitexact(wedge_mesh):
+    case = C.reference_wedge_case(wedge_mesh)
+    od, _ = oracle_for(case)
+    with C.make_context(case, L.FP64, L.FLAG_EXACT) as ctx:
+        g = ctx.geometry()
+        go = od.geometry()
+        for k in ("xf", "xc", "rf", "nf", "Af", "If", "iv"):
+            assert np.array_equal(g[k], go[k]), k
+        off, con = ctx.cf()
+        oo, oc = od.cf()
+        assert np.array_equal(off, oo) and np.array_equal(con, oc)
+
+
+def test_wedge_initialise_bitexact(wedge_mesh):
+    case = C.reference_wedge_case(wedge_mesh)
+    od, U0 = oracle_for(case)
+    with C.make_context(case, L.FP64, L.FLAG_EXACT) as ctx:
+        assert np.array_equal(ctx.get_state(), U0)
+        Q = ctx.get_prims()
+        assert np.array_equal(Q, od.prims(U0))
+        _, B0 = read_hsln(f"{GOLDEN}/Base_0.hsln")
+        assert relerr(Q, B0).max() < 1e-15
+
+
+def test_wedge_stages_bitexact(wedge_mesh):
+    """Each stage kernel against the matching Base method, on a developed state."""
+    case = C.reference_wedge_case(wedge_mesh)
+    od, U0 = oracle_for(case)
+    U, _ = od.run(U0, 50, dt=1e-3)
+    with C.make_context(case, L.FP64, L.FLAG_EXACT) as ctx:
+        ctx.stage_set(L.ARR_U, U)
+        ctx.stage("gradient")
+        G = od.gradient(U)
+        assert np.array_equal(ctx.stage_get(L.ARR_G), G)
+        ctx.stage("limiter")
+        phi = od.limiter(U, G)
+        assert np.array_equal(ctx.stage_get(L.ARR_PHI), phi)
+        ctx.stage("bc_grads")
+        G2 = od.bc_grads(G)
+        assert np.array_equal(ctx.stage_get(L.ARR_G), G2)
+        ctx.stage("bc_limiter")
+        phi2 = od.bc_limiter(phi)
+        assert np.array_equal(ctx.stage_get(L.ARR_PHI), phi2)
+        R0 = np.zeros((4, od.Nc))
+        ctx.stage_set(L.ARR_R, R0)
+        ctx.stage("flux")
+        R = od.flux(U, G2, phi2)
+        assert np.array_equal(ctx.stage_get(L.ARR_R), R)
+        ctx.stage("update", 1e-3)
+        U2 = U.copy()
+        R2 = R.copy()
+        od.lib.orc_update(od.h, 1e-3, ol._ptr(U2), ol._ptr(R2))
+        assert np.array_equal(ctx.stage_get(L.ARR_U), U2)
+        assert not ctx.stage_get(L.ARR_R).any()
+        ctx.stage("bc_cells")
+        assert np.array_equal(ctx.stage_get(L.ARR_U), od.bc_cells(U2))
+
+
+def test_wedge_reference_case_exact(wedge_mesh):
+    """src/Test.cpp's case: 1000 steps of dt = 1e-3, bit-identical to the oracle."""
+    case = C.reference_wedge_case(wedge_mesh)
+    od, U0 = oracle_for(case)
+    Uo, _ = od.run(U0, 1000, dt=1e-3)
+    with C.make_context(case, L.FP64, L.FLAG_EXACT) as ctx:
+        t = ctx.step(1000, dt=1e-3)
+        t_ref = 0.0
+        for _ in range(1000):
+            t_ref += 1e-3          # Euler::run's t += dt (src/Base/Euler.cpp:96)
+        assert t == t_ref
+        U = ctx.get_state()
+        assert np.array_equal(U, Uo)
+        _, B1 = read_hsln(f"{GOLDEN}/Base_1.hsln")
+        assert relerr(ctx.get_prims(), B1).max() < 1e-13
+        assert ctx.fault() == 0
+
+
+def test_wedge_reference_case_fast(wedge_mesh):
+    case = C.reference_wedge_case(wedge_mesh)
+    od, U0 = oracle_for(case)
+    Uo, _ = od.run(U0, 1000, dt=1e-3)
+    with C.make_context(case, L.FP64, 0) as ctx:
+        ctx.step(1000, dt=1e-3)
+        assert relerr(ctx.get_state(), Uo).max() < FP64_TOL
+
+
+@pytest.mark.parametrize("shuffle", [True, False])
+def test_riemann_cfl_exact(shuffle):
+    case = C.riemann_case(40, 20, seed=0, shuffle=shuffle)
+    od, U0 = oracle_for(case)
+    Uo, to = od.run(U0, 60, cfl=0.5)
+    with C.make_context(case, L.FP64, L.FLAG_EXACT) as ctx:
+        t = ctx.step(60, cfl=0.5)
+        assert t == to
+        assert np.array_equal(ctx.get_state(), Uo)
+
+
+def test_riemann_cfl_fast():
+    case = C.riemann_case(64, 32, seed=0)
+    od, U0 = oracle_for(case)
+    Uo, to = od.run(U0, 100, cfl=0.5)
+    with C.make_context(case, L.FP64, 0) as ctx:
+        t = ctx.step(100, cfl=0.5)
+        assert abs(t - to) / to < 1e-12
+        assert relerr(ctx.get_state(), Uo).max() < FP64_TOL
+
+
+def test_permutation_and_colouring():
+    case = C.riemann_case(30, 20, seed=3, shuffle=True)
+    hm = case.hmesh()
+    with C.make_context(case) as ctx:
+        inf = ctx.info()
+        p = ctx.permutation()
+        assert np.array_equal(np.sort(p), np.arange(hm.Nc))
+        ghost = hm.ghost_mask()
+        assert (p[ghost] >= inf.Nint).all() and (p[~ghost] < inf.Nint).all()
+        assert inf.bandwidth_out < inf.bandwidth_in
+        col, ncol = ctx.face_colours()
+        assert ncol == col.max() + 1 and col.min() >= 0
+        for c in range(ncol):
+            cells = hm.fc_con[col == c].ravel()
+            assert len(np.unique(cells)) == len(cells), f"colour {c} has two faces sharing a cell"
+
+
+def test_schedules_bitwise_identical():
+    case = C.riemann_case(50, 25, seed=1)
+    ref = None
+    for sched in ({}, dict(block=64, aosoa_width=8, persistent=4, xcd_remap=0),
+                  dict(block=128, aosoa_width=64, persistent=0, xcd_remap=1),
+                  dict(block=256, aosoa_width=1, persistent=8, xcd_remap=1)):
+        with C.make_context(case, L.FP64, 0) as ctx:
+            if sched:
+                ctx.set_schedule(**sched)
+            ctx.step(30, cfl=0.5)
+            U = ctx.get_state()
+        if ref is None:
+            ref = U
+        else:
+            assert np.array_equal(U, ref), sched
+
+
+def test_graph_replay_matches_eager():
+    case = C.riemann_case(40, 40, seed=2)
+    with C.make_context(case, L.FP64, 0) as a, C.make_context(case, L.FP64, L.FLAG_GRAPH) as b:
+        a.step(21, cfl=0.5)
+        b.step(21, cfl=0.5)
+        assert np.array_equal(a.get_state(), b.get_state())
+        a.step(10, dt=1e-5)
+        b.step(10, dt=1e-5)
+        assert np.array_equal(a.get_state(), b.get_state())
+
+
+def test_batch_matches_single_exact():
+    cs = [C.ramp_case(24, 12, 2.0, 10.0, seed=1), C.ramp_case(40, 20, 3.0, 20.0, seed=2),
+          C.ramp_case(16, 8, 1.8, 5.0, seed=3)]
+    with C.make_batch(cs, L.FP64, L.FLAG_EXACT) as b:
+        b.step(40, cfl=0.5)
+        tb, dtb, fb = b.status()
+        for k, c in enumerate(cs):
+            od, U0 = oracle_for(c)
+            Uo, to = od.run(U0, 40, cfl=0.5)
+            assert np.array_equal(b.inst_state(k), Uo), k
+            assert tb[k] == to and fb[k] == 0
+
+
+def test_mixed_precision_error_bound():
+    case = C.riemann_case(100, 50, seed=0)
+    with C.make_context(case, L.FP64, 0) as a, C.make_context(case, L.MIXED, 0) as m:
+        a.step(100, cfl=0.5)
+        m.step(100, cfl=0.5)
+        Ua, Um = a.get_state(), m.get_state()
+    e = relerr(Um, Ua)
+    assert e[0] < 1e-5, e
+    assert e.max() < 1e-4, e

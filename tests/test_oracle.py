@@ -1,0 +1,338 @@
+"""CPU tests that pin the oracle before it is trusted.
+
+* golden vectors shipped by the reference (sols/Base_0.hsln, sols/Base_1.hsln on
+  wedge.hmsh, the case of src/Test.cpp:51-74);
+* bit-exact agreement with the reference's own Base sources compiled under
+  oracle/_ref (skipped where /root/reference was not available to build them);
+* closed forms and invariants for what the golden case does not exercise
+  (SubInlet/SubOutlet/Wall ghosts, the CFL step, linear-field gradients,
+  conservation, Rusanov consistency and symmetry).
+A planted mistake in the oracle (a dropped term, a sign, a swapped index)
+fails at least one of these.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from cfd_optimisations_amd.formats import read_hsln, read_hmsh
+import oracle_lib as ol
+from conftest import GOLDEN
+
+G = 1.4
+
+
+def wedge_setup(m):
+    p = 1.0 / (G * 4.0)
+    qic = np.array([1.0, 1.0, 0.0, p])
+    bcmap = [0, 0, 0, 2, 4]
+    bct = np.array([bcmap[m.coltag[m.Ncol - m.NBCcol + b]] for b in range(m.NBCcol)])
+    return qic, np.tile(qic, (m.NBCcol, 1)), bct, bcmap
+
+
+def relerr(a, b):
+    return np.array([np.linalg.norm(a[v] - b[v]) / np.linalg.norm(b[v]) for v in range(4)])
+
+
+@pytest.fixture(scope="module")
+def wedge(wedge_mesh):
+    qic, qbc, bct, bcmap = wedge_setup(wedge_mesh)
+    od = ol.OracleDomain(wedge_mesh, G, bct, qbc)
+    return wedge_mesh, od, qic, qbc, bct, bcmap
+
+
+def test_golden_base0_base1(wedge):
+    m, od, qic, qbc, bct, _ = wedge
+    geo_before = od.geometry()["iv"].copy()
+    od.zero_bc_iv()
+    assert (od.geometry()["iv"][m.ghost_mask()] == 0).all() and (geo_before[m.ghost_mask()] != 0).all()
+    U0 = od.initialise(qic)
+    _, B0 = read_hsln(os.path.join(GOLDEN, "Base_0.hsln"))
+    assert relerr(od.prims(U0), B0).max() < 1e-15
+    U1, _ = od.run(U0, 1000, dt=1e-3)
+    _, B1 = read_hsln(os.path.join(GOLDEN, "Base_1.hsln"))
+    # FMA contraction in the shipped build vs -ffp-contract=off here: 3.7e-14 spread
+    assert relerr(od.prims(U1), B1).max() < 1e-13
+    # SoA_1 is bit-identical to Base_1 in the reference
+    _, S1 = read_hsln(os.path.join(GOLDEN, "SoA_1.hsln"))
+    assert np.array_equal(S1, B1)
+
+
+def test_hsln_file_layout():
+    names, Q = read_hsln(os.path.join(GOLDEN, "Base_1.hsln"))
+    assert names == ["Density", "Velocity_X", "Velocity_Y", "Pressure"]
+    assert Q.shape == (4, 3740)
+    assert os.path.getsize(os.path.join(GOLDEN, "Base_1.hsln")) == 119731
+
+
+need_ref = pytest.mark.skipif(not ol.ref_available(ol.REF_NOFMA_SO), reason="oracle/_ref not built (no /root/reference)")
+
+
+@need_ref
+def test_reference_build_reproduces_golden(wedge, wedge_path):
+    """The reference's own sources, built as its golden files were, reproduce them bit for bit."""
+    m, od, qic, qbc, bct, bcmap = wedge
+    rc = ol.RefCase(wedge_path, G, bcmap, qbc, ol.REF_SO)
+    rc.zero_bc_iv()
+    U = rc.run(rc.initialise(qic), 1000, 1e-3)
+    _, B1 = read_hsln(os.path.join(GOLDEN, "Base_1.hsln"))
+    assert np.array_equal(rc.prims(U), B1)
+
+
+def _compare_stages(m, path, bct_by_tag, qbc, qic, steps=40, dt=1e-3):
+    bct = np.array([bct_by_tag[m.coltag[m.Ncol - m.NBCcol + b]] for b in range(m.NBCcol)])
+    od = ol.OracleDomain(m, G, bct, qbc)
+    rc = ol.RefCase(path, G, bct_by_tag, qbc, ol.REF_NOFMA_SO)
+    go, gr = od.geometry(), rc.geometry()
+    for k in go:
+        assert np.array_equal(go[k], gr[k]), k
+    for a, b in zip(od.cf(), rc.cf()):
+        assert np.array_equal(a, b)
+    od.zero_bc_iv()
+    rc.zero_bc_iv()
+    U0 = od.initialise(qic)
+    assert np.array_equal(U0, rc.initialise(qic))
+    U = od.run(U0, steps, dt=dt)[0]
+    assert np.array_equal(U, rc.run(U0, steps, dt))
+    Gr = od.gradient(U)
+    assert np.array_equal(Gr, rc.gradient(U))
+    phi = od.limiter(U, Gr)
+    assert np.array_equal(phi, rc.limiter(U, Gr))
+    G2, p2 = od.bc_grads(Gr), od.bc_limiter(phi)
+    assert np.array_equal(G2, rc.bc_grads(Gr)) and np.array_equal(p2, rc.bc_limiter(phi))
+    R0 = np.random.default_rng(0).standard_normal((4, m.Nc))
+    assert np.array_equal(od.flux(U, G2, p2, R0), rc.flux(U, G2, p2, R0))
+    assert np.array_equal(od.bc_cells(U), rc.bc_cells(U))
+
+
+@need_ref
+def test_oracle_bitexact_vs_reference_wedge(wedge, wedge_path):
+    m, od, qic, qbc, bct, bcmap = wedge
+    _compare_stages(m, wedge_path, bcmap, qbc, qic, steps=100)
+
+
+@need_ref
+@pytest.mark.parametrize("bcset", [(0, 4, 0, 2), (1, 4, 1, 3), (0, 4, 1, 2)])
+def test_oracle_bitexact_vs_reference_generated(tmp_path, bcset):
+    """Ramp meshes with every BC type the reference implements correctly (not SubOutlet, Q3)."""
+    from cfd_optimisations_amd import lib as L
+    nm = L.NativeMesh.ramp(24, 12, 1.0, 0.5, 0.2, 12.0, 0.2, 7, True)
+    path = str(tmp_path / "r.hmsh")
+    nm.save(path)
+    m = read_hmsh(path)
+    q = np.array([1.0, 2.2, 0.1, 1.0 / G])
+    qbc = np.tile(q, (4, 1))
+    qbc[3] = [1.1, 2.0, 0.0, 0.8]
+    bymap = [0] + list(bcset)
+    _compare_stages(m, path, bymap, qbc, q, steps=60, dt=2e-4)
+
+
+# ---------------------------------------------------------------------------
+# closed forms
+# ---------------------------------------------------------------------------
+def euler_flux_np(U, n):
+    rho, mx, my, E = U
+    u, v = mx / rho, my / rho
+    p = (G - 1) * (E - 0.5 * rho * (u * u + v * v))
+    un = u * n[0] + v * n[1]
+    return np.array([rho * un, mx * un + p * n[0], my * un + p * n[1], (E + p) * un])
+
+
+def call_flux(name, *arrays):
+    lib = ol.oracle_lib()
+    F = np.zeros(4)
+    args = [np.ascontiguousarray(a, dtype=np.float64) for a in arrays]
+    getattr(lib, name)(G, *[ol._ptr(a) for a in args], ol._ptr(F))
+    return F
+
+
+def test_euler_flux_closed_form():
+    rng = np.random.default_rng(1)
+    for _ in range(50):
+        rho, u, v, p = rng.uniform(0.1, 2), rng.uniform(-3, 3), rng.uniform(-3, 3), rng.uniform(0.1, 3)
+        U = np.array([rho, rho * u, rho * v, p / (G - 1) + 0.5 * rho * (u * u + v * v)])
+        th = rng.uniform(0, 2 * np.pi)
+        n = np.array([math.cos(th), math.sin(th)])
+        F = call_flux("orc_euler_flux", U, n)
+        np.testing.assert_allclose(F, euler_flux_np(U, n), rtol=1e-13, atol=1e-13)
+
+
+def test_rusanov_consistency_and_symmetry():
+    rng = np.random.default_rng(2)
+    for _ in range(50):
+        def state():
+            rho, u, v, p = rng.uniform(0.1, 2), rng.uniform(-3, 3), rng.uniform(-3, 3), rng.uniform(0.1, 3)
+            return np.array([rho, rho * u, rho * v, p / (G - 1) + 0.5 * rho * (u * u + v * v)])
+        UL, UR = state(), state()
+        th = rng.uniform(0, 2 * np.pi)
+        n = np.array([math.cos(th), math.sin(th)])
+        # F(U, U, n) = E(U).n exactly
+        assert np.array_equal(call_flux("orc_rusanov", UL, UL, n), call_flux("orc_euler_flux", UL, n))
+        # F(UL, UR, n) = -F(UR, UL, -n)
+        np.testing.assert_allclose(call_flux("orc_rusanov", UL, UR, n), -call_flux("orc_rusanov", UR, UL, -n),
+                                   rtol=1e-13, atol=1e-13)
+        # Rusanov = average flux minus S/2 (UR - UL) with S the largest |u.n| +- c
+        def speed(U):
+            u = U[1:3] / U[0]
+            p = (G - 1) * (U[3] - 0.5 * U[0] * (u @ u))
+            return abs(u @ n) + math.sqrt(G * p / U[0])
+        S = max(speed(UL), speed(UR))
+        Fx = 0.5 * (euler_flux_np(UL, n) + euler_flux_np(UR, n)) - 0.5 * S * (UR - UL)
+        np.testing.assert_allclose(call_flux("orc_rusanov", UL, UR, n), Fx, rtol=1e-12, atol=1e-12)
+
+
+def uniform_grid(nx, ny, jitter=0.0, shuffle=False):
+    from cfd_optimisations_amd import lib as L
+    return L.NativeMesh.riemann(nx, ny, jitter, 0, shuffle).to_hmesh()
+
+
+def test_geometry_closed_forms():
+    m = uniform_grid(6, 4, jitter=0.2, shuffle=True)
+    od = ol.OracleDomain(m)
+    g = od.geometry()
+    interior = ~m.ghost_mask()
+    assert abs((1.0 / g["iv"][interior]).sum() - 1.0) < 1e-14            # total area of [0,1]^2
+    # closed cells: sum over faces of +-A n = 0
+    s = np.zeros((m.Nc, 2))
+    An = g["Af"][:, None] * g["nf"]
+    np.add.at(s, m.fc_con[:, 0], An)
+    np.add.at(s, m.fc_con[:, 1], -An)
+    assert np.abs(s[interior]).max() < 1e-15
+    # normals are unit and point from L to R
+    assert np.allclose(np.hypot(g["nf"][:, 0], g["nf"][:, 1]), 1.0, atol=1e-15)
+    assert (np.einsum("fd,fd->f", g["nf"], g["rf"][:, 0]) > 0).all()
+
+
+def test_gradient_exact_for_linear_fields():
+    """With dL = dR (uniform right-triangle grid) Green-Gauss is exact on cells with only interior faces."""
+    m = uniform_grid(8, 8)
+    od = ol.OracleDomain(m)
+    od.zero_bc_iv()
+    xc = od.geometry()["xc"]
+    a, b, c = 0.7, -1.3, 2.0
+    U = np.tile(a * xc[:, 0] + b * xc[:, 1] + c, (4, 1))
+    Gr = od.gradient(U)
+    bc_cells = set(m.fc_con[m.F_BC:, :].ravel())
+    inner = np.array([i for i in range(m.Nc) if i not in bc_cells])
+    np.testing.assert_allclose(Gr[:, 0, inner], a, atol=1e-12)
+    np.testing.assert_allclose(Gr[:, 1, inner], b, atol=1e-12)
+
+
+def test_limiter_bounds_and_uniform_state():
+    m = uniform_grid(10, 6, jitter=0.2, shuffle=True)
+    od = ol.OracleDomain(m)
+    od.zero_bc_iv()
+    U = np.tile(np.array([[1.0], [0.5], [0.1], [2.0]]), (1, m.Nc))
+    phi = od.limiter(U, od.gradient(U))
+    assert (phi == 1.0).all()
+    U = 1.0 + np.random.default_rng(3).random((4, m.Nc))
+    phi = od.limiter(U, od.gradient(U))
+    assert (phi >= 0).all() and (phi <= 1).all() and (phi < 1).any()
+
+
+def test_free_stream_preservation_and_conservation():
+    m = uniform_grid(12, 8, jitter=0.2, shuffle=True)
+    q = np.array([1.0, 1.5, -0.3, 1.0 / G])
+    od = ol.OracleDomain(m, G, np.full(4, 4), np.tile(q, (4, 1)))
+    od.zero_bc_iv()
+    U = od.initialise(q)
+    Gr = od.bc_grads(od.gradient(U))
+    phi = od.bc_limiter(od.limiter(U, Gr))
+    R = od.flux(U, Gr, phi)
+    assert np.abs(R).max() < 1e-11
+    # conservation: interior volume-weighted residual = sum of boundary-face fluxes
+    U = od.initialise(q) * (1 + 0.05 * np.random.default_rng(4).random((4, m.Nc)))
+    U = od.bc_cells(U)
+    Gr = od.bc_grads(od.gradient(U))
+    phi = od.bc_limiter(od.limiter(U, Gr))
+    R = od.flux(U, Gr, phi)
+    g = od.geometry()
+    iv = g["iv"]
+    interior = iv != 0
+    vol_sum = (R[:, interior] / iv[interior]).sum(axis=1)
+    # face fluxes on boundary faces, recomputed with the same reconstruction
+    bsum = np.zeros(4)
+    for f in range(m.F_BC, m.Nf):
+        L_, R_ = m.fc_con[f]
+        UfL = U[:, L_] + phi[:, L_] * (Gr[:, 0, L_] * g["rf"][f, 0, 0] + Gr[:, 1, L_] * g["rf"][f, 0, 1])
+        UfR = U[:, R_] + phi[:, R_] * (Gr[:, 0, R_] * g["rf"][f, 1, 0] + Gr[:, 1, R_] * g["rf"][f, 1, 1])
+        bsum -= g["Af"][f] * call_flux("orc_rusanov", UfL, UfR, g["nf"][f])
+    np.testing.assert_allclose(vol_sum, bsum, rtol=1e-12, atol=1e-13)
+
+
+def test_bc_ghost_closed_forms():
+    m = uniform_grid(4, 3)
+    rng = np.random.default_rng(5)
+    qbc = np.array([[1.2, 0.3, -0.2, 0.9]] * 4)
+    interior = ~m.ghost_mask()
+    for bt in range(6):
+        od = ol.OracleDomain(m, G, np.full(4, bt), qbc)
+        g = od.geometry()
+        U = np.vstack([1 + rng.random(m.Nc), rng.standard_normal(m.Nc), rng.standard_normal(m.Nc), 5 + rng.random(m.Nc)])
+        V = od.bc_cells(U)
+        assert np.array_equal(V[:, interior], U[:, interior])
+        for f in range(m.F_BC, m.Nf):
+            L_, R_ = m.fc_con[f]
+            uL, uG = U[:, L_], V[:, R_]
+            n = g["nf"][f]
+            q = qbc[0]
+            UBC = np.array([q[0], q[0] * q[1], q[0] * q[2], q[3] / (G - 1) + 0.5 * q[0] * (q[1] ** 2 + q[2] ** 2)])
+            pL = (G - 1) * (uL[3] - 0.5 * (uL[1] ** 2 + uL[2] ** 2) / uL[0])
+            if bt == 0:    # symmetry: |u| kept, normal component flipped
+                assert uG[0] == uL[0] and uG[3] == uL[3]
+                assert abs(np.hypot(uG[1], uG[2]) - np.hypot(uL[1], uL[2])) < 1e-14
+                assert abs(uG[1:3] @ n + uL[1:3] @ n) < 1e-14
+                t = np.array([-n[1], n[0]])
+                assert abs(uG[1:3] @ t - uL[1:3] @ t) < 1e-14
+            elif bt == 1:  # wall: momentum reversed
+                assert np.array_equal(uG, uL * np.array([1, -1, -1, 1]))
+            elif bt == 2:  # supersonic inlet: UBC
+                assert np.array_equal(uG, UBC)
+            elif bt == 3:  # subsonic inlet: UBC with the interior pressure
+                assert np.array_equal(uG[:3], UBC[:3])
+                assert abs((G - 1) * (uG[3] - 0.5 * q[0] * (q[1] ** 2 + q[2] ** 2)) - pL) < 1e-13
+            elif bt == 4:  # supersonic outlet: copy
+                assert np.array_equal(uG, uL)
+            else:          # subsonic outlet: interior state with the BC pressure
+                assert np.array_equal(uG[:3], uL[:3])
+                pG = (G - 1) * (uG[3] - 0.5 * (uG[1] ** 2 + uG[2] ** 2) / uG[0])
+                assert abs(pG - q[3]) < 1e-13
+
+
+def test_cfl_dt_closed_form():
+    """Uniform right-triangle grid, uniform state (rho=1, u=(1,0), p=1/gamma so c=1):
+    every triangle has lam = h(0+1) + h(1+1) + h sqrt2 (1/sqrt2 + 1), V = h^2/2."""
+    n = 10
+    m = uniform_grid(n, n)
+    q = np.array([1.0, 1.0, 0.0, 1.0 / G])
+    od = ol.OracleDomain(m, G, np.full(4, 4), np.tile(q, (4, 1)))
+    od.zero_bc_iv()
+    U = od.initialise(q)
+    h = 1.0 / n
+    expect = 0.5 * (h * h / 2) / (4 * h + math.sqrt(2) * h)
+    assert abs(od.cfl_dt(U, 0.5) - expect) < 1e-15
+    # brute force over cells on a jittered mesh
+    m = uniform_grid(5, 4, jitter=0.2, shuffle=True)
+    od = ol.OracleDomain(m, G, np.full(4, 4), np.tile(q, (4, 1)))
+    od.zero_bc_iv()
+    rng = np.random.default_rng(6)
+    U = od.initialise(q) * (1 + 0.1 * rng.random((4, m.Nc)))
+    g = od.geometry()
+    best = np.inf
+    for i in range(m.Nc):
+        if g["iv"][i] == 0:
+            continue
+        rho, u, v = U[0, i], U[1, i] / U[0, i], U[2, i] / U[0, i]
+        p = (G - 1) * (U[3, i] - 0.5 * rho * (u * u + v * v))
+        c = math.sqrt(G * p / rho)
+        faces = np.where((m.fc_con[:, 0] == i) | (m.fc_con[:, 1] == i))[0]
+        lam = sum(g["Af"][f] * (abs(u * g["nf"][f, 0] + v * g["nf"][f, 1]) + c) for f in faces)
+        best = min(best, (1 / g["iv"][i]) / lam)
+    assert abs(od.cfl_dt(U, 0.7) - 0.7 * best) < 1e-14 * best
+
+
+def test_step_count_semantics():
+    lib = ol.oracle_lib()
+    assert lib.orc_count_steps(1.0, 1e-3) == 1000
+    assert lib.orc_count_steps(0.0, 1e-3) == 0
