@@ -118,14 +118,15 @@ def build_case(name: str):
     raise SystemExit(f"unknown workload {name}")
 
 
-def run_single(name: str, steps: int, warmup: int, cpu_s: float, graph: bool, secondary: bool, tune: bool):
+def run_single(name: str, steps: int, warmup: int, cpu_s: float, graph: bool, secondary: bool, tune: bool,
+               order: str = "sfc"):
     import numpy as np
     from cfd_optimisations_amd import cases as C
     from cfd_optimisations_amd import lib as L
     case, w, prec, desc = build_case(name)
     steps = steps or w["steps"]
     cfl = w["cfl"]
-    flags = L.FLAG_GRAPH if graph else 0
+    flags = (L.FLAG_GRAPH if graph else 0) | (L.FLAG_ORDER_SFC if order == "sfc" else 0)
     t0 = time.perf_counter()
     ctx = C.make_context(case, prec, flags)
     setup_s = time.perf_counter() - t0
@@ -150,7 +151,7 @@ def run_single(name: str, steps: int, warmup: int, cpu_s: float, graph: bool, se
         "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32+f64" if prec == L.MIXED else "f64", "data": "synthetic",
         "config": {"workload": desc, "Nc": inf.Nc, "Nint": inf.Nint, "Nf": inf.Nf, "steps": steps, "cfl": cfl,
-                   "schedule": ctx.schedule(), "graph": graph, "setup_s": round(setup_s, 2),
+                   "schedule": ctx.schedule(), "graph": graph, "setup_s": round(setup_s, 2), "cell_order": order,
                    "rcm_bandwidth": [inf.bandwidth_in, inf.bandwidth_out], "face_colours": inf.ncolours},
         "hbm_pct_step": 100.0 * step_b / (ms_step * 1e-3) / (HBM_PEAK_GBS * 1e9),
         "alg_bytes_per_step": step_b,
@@ -160,7 +161,9 @@ def run_single(name: str, steps: int, warmup: int, cpu_s: float, graph: bool, se
                      "frac": achieved / HBM_PEAK_GBS, "alg_bytes": dbytes,
                      "traffic": (traffic or {}).get(dom)},
         "fault": ctx.fault(),
+        "copy_ceiling_gbs": L.copy_bandwidth(1 << 30, 20),
     }
+    line["roofline"]["frac_of_copy_ceiling"] = achieved / line["copy_ceiling_gbs"]
     if cpu_s > 0:
         dt = ctx.last_dt()
         line["cpu_baseline"] = cpu_baseline(case, ctx, dt, cpu_s)
@@ -215,6 +218,8 @@ def main(argv=None) -> int:
     ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c3mixed"])
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample length (0 = skip)")
     ap.add_argument("--graph", action="store_true")
+    ap.add_argument("--order", default="sfc", choices=["sfc", "rcm"],
+                    help="interior cell order: Hilbert curve of centroids (sfc) or reverse Cuthill-McKee (rcm)")
     ap.add_argument("--tune", action="store_true", help="run the autotuner before timing")
     ap.add_argument("--no-secondary", action="store_true")
     a = ap.parse_args(argv)
@@ -222,7 +227,7 @@ def main(argv=None) -> int:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus}).",
               file=sys.stderr)
         return 2
-    line = run_single(a.workload, a.steps, a.warmup, a.cpu_seconds, a.graph, not a.no_secondary, a.tune)
+    line = run_single(a.workload, a.steps, a.warmup, a.cpu_seconds, a.graph, not a.no_secondary, a.tune, a.order)
     print(json.dumps(line))
     return 0
 

@@ -84,9 +84,9 @@ struct hcfd_ctx {
     int* d_ghost_inst = nullptr;
     double* U[2] = {nullptr, nullptr};
     int cur = 0;
-    double* G = nullptr;     // phi*G (fast) or G (exact), 8 per interior cell
-    double* PHI = nullptr;   // exact mode
-    float* G32 = nullptr;    // mixed
+    double* FS = nullptr;    // FP64 face states [NcPad][deg_bucket][4]
+    float* FS32 = nullptr;   // FP32-mixed face states
+    FaceN32* fn32 = nullptr;
     FaceRec32* faces32 = nullptr;
     float* iv32 = nullptr;
     double* pmin_first = nullptr;
@@ -122,7 +122,7 @@ void free_ptr(void* p) {
 
 std::string sched_key(const hcfd_schedule& s) {
     char b[96];
-    snprintf(b, sizeof b, "%d/%d/%d/%d/%d", s.block, s.cells_per_thread, s.aosoa_width, s.persistent, s.xcd_remap);
+    snprintf(b, sizeof b, "%d/%d/%d/%d/%d", s.block, s.lanes_per_cell, s.aosoa_width, s.persistent, s.xcd_remap);
     return b;
 }
 
@@ -232,7 +232,7 @@ hcfd_ctx* create_ctx(int n, const hcfd_mesh_view* views, int precision, int flag
     in.bcid = bcid.data();
     in.ghost = ghost.data();
     in.inst_cell_off = c->inst_cell_off.data();
-    in.renumber = !(flags & HCFD_FLAG_NO_RENUMBER);
+    in.order = (flags & HCFD_FLAG_NO_RENUMBER) ? ORDER_CALLER : (flags & HCFD_FLAG_ORDER_SFC) ? ORDER_SFC : ORDER_RCM;
     build_device_mesh(in, c->m, c->st);
 
     c->Nc = c->m.Nc;
@@ -265,17 +265,20 @@ hcfd_ctx* create_ctx(int n, const hcfd_mesh_view* views, int precision, int flag
     c->U[1] = dmalloc<double>(4 * P);
     HIPCHK(hipMemsetAsync(c->U[0], 0, sizeof(double) * 4 * P, c->st));
     HIPCHK(hipMemsetAsync(c->U[1], 0, sizeof(double) * 4 * P, c->st));
+    // face states, cell-major: [NcPad][deg_bucket slots][4]
+    const size_t nfs = P * (size_t)c->deg_bucket * NV;
     if (precision == HCFD_FP64) {
-        c->G = dmalloc<double>(8 * P);
-        if (flags & HCFD_FLAG_EXACT) c->PHI = dmalloc<double>(4 * P);
+        c->FS = dmalloc<double>(nfs);
     } else {
-        c->G32 = dmalloc<float>(8 * P);
+        c->FS32 = dmalloc<float>(nfs);
         c->faces32 = dmalloc<FaceRec32>(c->Nf);
+        c->fn32 = dmalloc<FaceN32>(c->Nf);
         c->iv32 = dmalloc<float>(P);
         // FP32 geometry from the FP64 records (same values rounded)
         HIPCHK(launch_geom32(c->Nf, c->m.faces, c->faces32, (int)P, c->m.iv, c->iv32, c->st));
+        HIPCHK(launch_fn32(c->Nf, c->m.fn, c->fn32, c->st));
     }
-    const int maxtiles = (c->Nint + WAVE - 1) / WAVE + 1;
+    const int maxtiles = (c->Nint + 15) / 16 + 1;  // smallest pass-A tile: 64 threads / 4 lanes per cell
     c->pmin_first = dmalloc<double>(maxtiles);
     c->pmin_last = dmalloc<double>(maxtiles);
     c->dtacc = dmalloc<unsigned long long>(n);
@@ -362,20 +365,21 @@ struct Launch {
     int grid = 0, block = 0;
 };
 
-Launch make_launch(hcfd_ctx* c, bool cfl, int cur) {
+// pass: 0 = pass A (tiles of block / lanes_per_cell cells), 1 = pass B (one cell per lane).
+Launch make_launch(hcfd_ctx* c, bool cfl, int cur, int pass) {
     Launch L;
     const hcfd_schedule& s = c->sched;
     L.v.precision = c->precision;
-    L.v.exact = (c->flags & HCFD_FLAG_EXACT) != 0;
     L.v.cfl = cfl;
     L.v.width = s.aosoa_width;
     L.v.maxdeg = c->deg_bucket;
+    L.v.lpc = pass == 0 ? s.lanes_per_cell : 1;
     PassArgs& a = L.a;
     a.Nint = c->Nint;
     a.Nc = c->Nc;
     a.NcPad = c->NcPad;
     a.ninst = c->ninst;
-    a.tile_cells = s.block;
+    a.tile_cells = s.block / L.v.lpc;
     a.ntiles = (c->Nint + a.tile_cells - 1) / a.tile_cells;
     a.xcd_remap = s.xcd_remap;
     a.tiles_per_xcd = (a.ntiles + NXCD - 1) / NXCD;
@@ -383,13 +387,14 @@ Launch make_launch(hcfd_ctx* c, bool cfl, int cur) {
     a.slots = c->m.slots;
     a.faces = c->m.faces;
     a.faces32 = c->faces32;
+    a.fn = c->m.fn;
+    a.fn32 = c->fn32;
     a.iv = c->m.iv;
     a.iv32 = c->iv32;
     a.U = c->U[cur];
     a.Unext = c->U[cur ^ 1];
-    a.G = c->G;
-    a.PHI = c->PHI;
-    a.G32 = c->G32;
+    a.FS = c->FS;
+    a.FS32 = c->FS32;
     a.inst_off = c->m.inst_int_off;
     a.pmin_first = c->pmin_first;
     a.pmin_last = c->pmin_last;
@@ -420,13 +425,14 @@ void load_fixed_dt(hcfd_ctx* c, double dt) {
 // Enqueues one step reading U[cur]; returns nothing, caller flips cur.
 void enqueue_step(hcfd_ctx* c, int cur, double cfl, hipEvent_t* ev = nullptr) {
     const bool use_cfl = cfl > 0;
-    Launch L = make_launch(c, use_cfl, cur);
+    Launch L = make_launch(c, use_cfl, cur, 0);
+    Launch LB = make_launch(c, use_cfl, cur, 1);
     if (ev) HIPCHK(hipEventRecord(ev[0], c->st));
     HIPCHK(launch_pass_a(L.v, L.a, L.grid, L.block, c->st));
     if (ev) HIPCHK(hipEventRecord(ev[1], c->st));
     if (use_cfl) HIPCHK(launch_dt_reduce(L.a, cfl, c->dt, c->tdev, c->lastdt, c->st));
     if (ev) HIPCHK(hipEventRecord(ev[2], c->st));
-    HIPCHK(launch_pass_b(L.v, L.a, L.grid, L.block, c->st));
+    HIPCHK(launch_pass_b(LB.v, LB.a, LB.grid, LB.block, c->st));
     if (ev) HIPCHK(hipEventRecord(ev[3], c->st));
 }
 
@@ -540,7 +546,7 @@ void change_width(hcfd_ctx* c, int neww) {
 
 void check_schedule(const hcfd_ctx* c, const hcfd_schedule& s) {
     if (s.block != 64 && s.block != 128 && s.block != 256) fail(HCFD_ERR_ARG, "block must be 64, 128 or 256");
-    if (s.cells_per_thread != 1) fail(HCFD_ERR_ARG, "cells_per_thread must be 1");
+    if (s.lanes_per_cell != 1 && s.lanes_per_cell != 4) fail(HCFD_ERR_ARG, "lanes_per_cell must be 1 or 4");
     if (s.aosoa_width != 1 && s.aosoa_width != 8 && s.aosoa_width != 64)
         fail(HCFD_ERR_ARG, "aosoa_width must be 1, 8 or 64");
     if (s.persistent < 0 || s.persistent > 64) fail(HCFD_ERR_ARG, "persistent must be in [0, 64]");
@@ -553,8 +559,8 @@ hcfd_ctx::~hcfd_ctx() {
     if (st) (void)hipStreamSynchronize(st);
     if (gexec) (void)hipGraphExecDestroy(gexec);
     free_device_mesh(m);
-    for (void* p : {(void*)bctab, (void*)d_old_of_new, (void*)d_ghost_inst, (void*)U[0], (void*)U[1], (void*)G,
-                    (void*)PHI, (void*)G32, (void*)faces32, (void*)iv32, (void*)pmin_first, (void*)pmin_last,
+    for (void* p : {(void*)bctab, (void*)d_old_of_new, (void*)d_ghost_inst, (void*)U[0], (void*)U[1], (void*)FS,
+                    (void*)FS32, (void*)fn32, (void*)faces32, (void*)iv32, (void*)pmin_first, (void*)pmin_last,
                     (void*)dtacc, (void*)dt, (void*)tdev, (void*)lastdt, (void*)fault, (void*)staging, (void*)sU,
                     (void*)sG, (void*)sPhi, (void*)sR, (void*)backup})
         free_ptr(p);
@@ -635,10 +641,11 @@ int hcfd_get_info(hcfd_ctx* c, hcfd_info* o) {
         o->ncolours = c->m.ncolours;
         size_t P = c->NcPad;
         long long b = 0;
-        b += (long long)c->maxdeg * P * sizeof(Slot) + (long long)c->Nf * sizeof(FaceRec64) + P * 8;
+        b += (long long)c->maxdeg * P * sizeof(Slot) + P * 8;
+        b += (long long)c->Nf * (sizeof(FaceRec64) + sizeof(FaceN64));
         b += 8LL * P * 8;                         // U x2
-        b += (c->G ? 8LL * P * 8 : 0) + (c->PHI ? 4LL * P * 8 : 0) + (c->G32 ? 8LL * P * 4 : 0);
-        b += (c->faces32 ? (long long)c->Nf * sizeof(FaceRec32) + P * 4 : 0);
+        b += (c->FS ? 32LL : c->FS32 ? 16LL : 0) * (long long)P * c->deg_bucket;
+        b += (c->faces32 ? (long long)c->Nf * (sizeof(FaceRec32) + sizeof(FaceN32)) + P * 4 : 0);
         b += 8LL * c->Nc * 8 + 16LL * c->Nf + 16LL * c->Nc;  // staging, xf, xc
         o->device_bytes = b;
         o->sched = c->sched;
@@ -798,6 +805,32 @@ int hcfd_stage_flux(hcfd_ctx* c) { return guarded([&] { run_stage(c, STAGE_FLUX,
 int hcfd_stage_update(hcfd_ctx* c, double dt) { return guarded([&] { run_stage(c, STAGE_UPDATE, dt); }); }
 int hcfd_stage_bc_cells(hcfd_ctx* c) { return guarded([&] { run_stage(c, STAGE_BC_CELLS, 0); }); }
 
+int hcfd_stage_bc(hcfd_ctx* c, int what, int bc) {
+    return guarded([&] {
+        if (!c) fail(HCFD_ERR_ARG, "null");
+        if (c->ninst != 1) fail(HCFD_ERR_STATE, "per-BC stages apply to single-mesh contexts");
+        if (bc < -1 || bc >= c->inst_nbc[0]) fail(HCFD_ERR_ARG, "BC ordinal out of range");
+        const int which = what == 0 ? STAGE_BC_CELLS : what == 1 ? STAGE_BC_GRADS : what == 2 ? STAGE_BC_LIMITER : -1;
+        if (which < 0) fail(HCFD_ERR_ARG, "what must be 0 (cells), 1 (gradients) or 2 (limiter)");
+        ensure_stage_arrays(c);
+        if (which == STAGE_BC_CELLS) require_bcs(c);
+        StageArgs a = stage_args(c);
+        a.only_bc = bc;
+        HIPCHK(stage_launch(which, a, 0.0, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+    });
+}
+
+int hcfd_set_gamma(hcfd_ctx* c, double gamma) {
+    return guarded([&] {
+        if (!c) fail(HCFD_ERR_ARG, "null");
+        if (!(gamma > 1.0)) fail(HCFD_ERR_ARG, "gamma must exceed 1");
+        if (c->bcs_set && gamma != c->gamma) fail(HCFD_ERR_STATE, "gamma differs from the one the BCs were set with");
+        c->gamma = gamma;
+        drop_graph(c);
+    });
+}
+
 int hcfd_step(hcfd_ctx* c, int nsteps, double dt, double cfl, double* t_inout) {
     return guarded([&] {
         if (!c) fail(HCFD_ERR_ARG, "null");
@@ -865,6 +898,46 @@ int hcfd_time_steps(hcfd_ctx* c, int nsteps, double dt, double cfl, float* ms_ou
     });
 }
 
+int hcfd_copy_bandwidth(long long bytes, int reps, double* gbs) {
+    return guarded([&] {
+        if (bytes < 16 || reps <= 0 || !gbs) fail(HCFD_ERR_ARG, "bad argument");
+        const long long n16 = bytes / 16;
+        if ((n16 + 255) / 256 > 0x7fffffffLL) fail(HCFD_ERR_ARG, "copy too large");
+        void* a = nullptr;
+        void* b = nullptr;
+        hipStream_t st = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        auto cleanup = [&] {
+            if (a) (void)hipFree(a);
+            if (b) (void)hipFree(b);
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            if (st) (void)hipStreamDestroy(st);
+        };
+        try {
+            HIPCHK(hipMalloc(&a, n16 * 16));
+            HIPCHK(hipMalloc(&b, n16 * 16));
+            HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+            HIPCHK(hipMemsetAsync(a, 0, n16 * 16, st));
+            HIPCHK(hipEventCreate(&e0));
+            HIPCHK(hipEventCreate(&e1));
+            const int grid = (int)((n16 + 255) / 256);  // one element per lane
+            HIPCHK(launch_copy16(n16, a, b, grid, st));  // warm-up
+            HIPCHK(hipEventRecord(e0, st));
+            for (int r = 0; r < reps; r++) HIPCHK(launch_copy16(n16, a, b, grid, st));
+            HIPCHK(hipEventRecord(e1, st));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            *gbs = 2.0 * 16.0 * n16 * reps / (ms * 1e-3) / 1e9;
+        } catch (...) {
+            cleanup();
+            throw;
+        }
+        cleanup();
+    });
+}
+
 int hcfd_set_schedule(hcfd_ctx* c, const hcfd_schedule* s) {
     return guarded([&] {
         if (!c) fail(HCFD_ERR_ARG, "null");
@@ -899,7 +972,7 @@ int hcfd_autotune(hcfd_ctx* c, int steps, double cfl, const char* cache_path, hc
                 auto bar = line.find('|');
                 if (bar == std::string::npos || line.substr(0, bar) != key) continue;
                 hcfd_schedule s{};
-                if (sscanf(line.c_str() + bar + 1, "%d %d %d %d %d", &s.block, &s.cells_per_thread, &s.aosoa_width,
+                if (sscanf(line.c_str() + bar + 1, "%d %d %d %d %d", &s.block, &s.lanes_per_cell, &s.aosoa_width,
                            &s.persistent, &s.xcd_remap) == 5) {
                     check_schedule(c, s);
                     change_width(c, s.aosoa_width);
@@ -916,8 +989,9 @@ int hcfd_autotune(hcfd_ctx* c, int steps, double cfl, const char* cache_path, hc
         std::vector<hcfd_schedule> cands;
         for (int w : {1, 8, 64})
             for (int b : {64, 128, 256})
-                for (int pers : {0, 4, 8})
-                    for (int x : {0, 1}) cands.push_back(hcfd_schedule{b, 1, w, pers, x});
+                for (int lpc : {1, 4})
+                    for (int pers : {0, 8})
+                        for (int x : {0, 1}) cands.push_back(hcfd_schedule{b, lpc, w, pers, x});
         const double t_save = c->t_host;
         hipEvent_t e0, e1;
         HIPCHK(hipEventCreate(&e0));
@@ -962,7 +1036,7 @@ int hcfd_autotune(hcfd_ctx* c, int steps, double cfl, const char* cache_path, hc
         }
         if (cache_path) {
             std::ofstream out(cache_path, std::ios::app);
-            out << key << "|" << bs.block << " " << bs.cells_per_thread << " " << bs.aosoa_width << " "
+            out << key << "|" << bs.block << " " << bs.lanes_per_cell << " " << bs.aosoa_width << " "
                 << bs.persistent << " " << bs.xcd_remap << "  # " << best_ms / steps << " ms/step\n";
         }
         if (best) *best = bs;

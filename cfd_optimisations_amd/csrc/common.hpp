@@ -40,6 +40,21 @@ struct alignas(32) FaceRec32 {
 };
 static_assert(sizeof(FaceRec32) == 32, "FaceRec32 must be 32 bytes");
 
+// Flux geometry of a face for pass B, FP64 (32 bytes): A, unit normal L->R,
+// and the ghost cell (new numbering) of a boundary face, else -1.
+struct alignas(32) FaceN64 {
+    double A, n0, n1;
+    int ghost, pad;
+};
+static_assert(sizeof(FaceN64) == 32, "FaceN64 must be 32 bytes");
+
+// Same for the FP32-mixed path (16 bytes).
+struct alignas(16) FaceN32 {
+    float A, n0, n1;
+    int ghost;
+};
+static_assert(sizeof(FaceN32) == 16, "FaceN32 must be 16 bytes");
+
 // Boundary-condition table entry (one per BC ordinal per instance).
 struct BCEntry {
     int type;      // hcfd_bctype
@@ -48,11 +63,14 @@ struct BCEntry {
     double QBC[NV];
 };
 
-// Slot (cell -> face incidence): x = face id | side bit (or SLOT_EMPTY), y = neighbour cell.
+// Slot (cell -> face incidence): f = face id | side bit (or SLOT_EMPTY);
+// nb = neighbour cell | (slot of this face in the neighbour's list) << 29.
 struct Slot {
     int f;
     int nb;
 };
+constexpr int SLOT_TWIN_SHIFT = 29;
+constexpr int SLOT_NB_MASK = (1 << SLOT_TWIN_SHIFT) - 1;
 
 inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 inline long long round_up_ll(long long x, long long m) { return (x + m - 1) / m * m; }

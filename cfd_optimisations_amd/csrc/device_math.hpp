@@ -11,6 +11,9 @@
 
 namespace hcfd {
 
+__device__ __forceinline__ int slot_nb(const Slot& s) { return s.nb & SLOT_NB_MASK; }
+__device__ __forceinline__ int slot_twin(const Slot& s) { return (int)((unsigned)s.nb >> SLOT_TWIN_SHIFT); }
+
 template <typename T>
 __device__ __forceinline__ T dmin(T a, T b) { return fmin(a, b); }
 template <typename T>

@@ -30,6 +30,15 @@ hipError_t with_degree(int d, F&& f) {
 }
 
 template <typename F>
+hipError_t with_lanes(int lpc, F&& f) {
+    switch (lpc) {
+        case 1: return f(IC<1>{});
+        case 4: return f(IC<4>{});
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename F>
 hipError_t with_bool(bool b, F&& f) {
     if (b) return f(std::true_type{});
     return f(std::false_type{});

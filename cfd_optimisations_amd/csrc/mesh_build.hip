@@ -166,31 +166,61 @@ __global__ void k_interior_degree(int Nc, const int* __restrict__ off, const int
     ideg[i] = d;
 }
 
+__device__ __forceinline__ unsigned long long shfl_xor_u64(unsigned long long v, int m) {
+    const unsigned lo = __shfl_xor((unsigned)(v & 0xffffffffu), m, 64);
+    const unsigned hi = __shfl_xor((unsigned)(v >> 32), m, 64);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
 // Per instance: the unplaced interior cell with minimum key (degree, id), or
-// (when use_level) minimum (degree, id) among cells on the instance's deepest level.
+// (when use_level) minimum (degree, id) among cells on the instance's deepest
+// level.  A wave whose cells all belong to one instance reduces first and
+// issues one atomic.
 __global__ void k_pick_start(int Nc, const int* __restrict__ inst_off, int ninst,
                              const unsigned char* __restrict__ ghost, const int* __restrict__ ideg,
                              const int* __restrict__ cmpos, const int* __restrict__ level,
                              const int* __restrict__ inst_maxlevel, int use_level,
                              unsigned long long* __restrict__ best) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= Nc || ghost[i]) return;
-    int b = instance_of(inst_off, ninst, i);
-    if (use_level) {
-        if (level[i] != inst_maxlevel[b]) return;
-    } else {
-        if (cmpos[i] >= 0) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ic = min(i, Nc - 1);
+    const int b = instance_of(inst_off, ninst, ic);
+    bool ok = i < Nc && !ghost[i];
+    if (ok) ok = use_level ? (level[i] == inst_maxlevel[b]) : (cmpos[i] < 0);
+    unsigned long long key = ok ? (((unsigned long long)(unsigned)ideg[i] << 32) | (unsigned)i) : ~0ull;
+    int bmin = b, bmax = b;
+    for (int m = 32; m > 0; m >>= 1) {
+        bmin = min(bmin, __shfl_xor(bmin, m, 64));
+        bmax = max(bmax, __shfl_xor(bmax, m, 64));
     }
-    unsigned long long key = ((unsigned long long)(unsigned)ideg[i] << 32) | (unsigned)i;
-    atomicMin(&best[b], key);
+    if (bmin == bmax) {
+        for (int m = 32; m > 0; m >>= 1) {
+            const unsigned long long o = shfl_xor_u64(key, m);
+            key = o < key ? o : key;
+        }
+        if ((threadIdx.x & (WAVE - 1)) == 0 && key != ~0ull) atomicMin(&best[b], key);
+    } else if (ok) {
+        atomicMin(&best[b], key);
+    }
 }
 
 __global__ void k_max_level(int Nc, const int* __restrict__ inst_off, int ninst, const unsigned char* __restrict__ ghost,
                             const int* __restrict__ level, int* __restrict__ inst_maxlevel) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= Nc || ghost[i]) return;
-    int b = instance_of(inst_off, ninst, i);
-    atomicMax(&inst_maxlevel[b], level[i]);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ic = min(i, Nc - 1);
+    const int b = instance_of(inst_off, ninst, ic);
+    const bool ok = i < Nc && !ghost[i];
+    int lv = ok ? level[i] : -1;
+    int bmin = b, bmax = b;
+    for (int m = 32; m > 0; m >>= 1) {
+        bmin = min(bmin, __shfl_xor(bmin, m, 64));
+        bmax = max(bmax, __shfl_xor(bmax, m, 64));
+    }
+    if (bmin == bmax) {
+        for (int m = 32; m > 0; m >>= 1) lv = max(lv, __shfl_xor(lv, m, 64));
+        if ((threadIdx.x & (WAVE - 1)) == 0 && lv >= 0) atomicMax(&inst_maxlevel[b], lv);
+    } else if (ok) {
+        atomicMax(&inst_maxlevel[b], lv);
+    }
 }
 
 // Seeds the frontier with each instance's chosen start (if any) and places it.
@@ -309,6 +339,49 @@ __global__ void k_identity_keys(int Nc, const unsigned char* __restrict__ ghost,
     val[slot] = i;
 }
 
+// Index of (x, y) along a Hilbert curve of order 16 (the iterative rotate-and-
+// reflect construction of Hilbert's curve; 2^16 x 2^16 cells).
+__device__ __forceinline__ unsigned hilbert16(unsigned x, unsigned y) {
+    const unsigned n = 1u << 16;
+    unsigned d = 0;
+    for (unsigned s = n >> 1; s > 0; s >>= 1) {
+        const unsigned rx = (x & s) ? 1u : 0u;
+        const unsigned ry = (y & s) ? 1u : 0u;
+        d += s * s * ((3u * rx) ^ ry);
+        if (ry == 0) {  // rotate the quadrant
+            if (rx == 1) {
+                x = n - 1 - x;
+                y = n - 1 - y;
+            }
+            const unsigned t = x;
+            x = y;
+            y = t;
+        }
+    }
+    return d;
+}
+
+// Sort keys for a space-filling-curve order of the interior cells: (instance,
+// Hilbert index of the centroid in the instance's bounding box).
+__global__ void k_sfc_keys(int Nc, const int* __restrict__ inst_off, int ninst,
+                           const unsigned char* __restrict__ ghost, const double2* __restrict__ xc,
+                           const double4* __restrict__ bbox, unsigned long long* __restrict__ key,
+                           int* __restrict__ val, int* __restrict__ n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Nc || ghost[i]) return;
+    const int b = instance_of(inst_off, ninst, i);
+    const double4 bb = bbox[b];  // xmin, ymin, 1/(xmax-xmin), 1/(ymax-ymin)
+    const double2 p = xc[i];
+    double fx = (p.x - bb.x) * bb.z, fy = (p.y - bb.y) * bb.w;
+    fx = fmin(fmax(fx, 0.0), 1.0);
+    fy = fmin(fmax(fy, 0.0), 1.0);
+    const unsigned qx = min((unsigned)(fx * 65536.0), 65535u);
+    const unsigned qy = min((unsigned)(fy * 65536.0), 65535u);
+    int slot = atomicAdd(n, 1);
+    key[slot] = ((unsigned long long)(unsigned)b << 32) | hilbert16(qx, qy);
+    val[slot] = i;
+}
+
 __global__ void k_scatter_rank(int n, const int* __restrict__ sorted_val, int base, int* __restrict__ new_of_old) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
@@ -380,10 +453,41 @@ __global__ void k_pack_slots(int Nc, int NcPad, int maxdeg, const int* __restric
             bool isR = (c.x != o);
             int j = isR ? c.x : c.y;
             s.f = face_new_of_old[f] | (isR ? SLOT_SIDE_R : 0);
-            s.nb = new_of_old[j];
+            // position of this face in the neighbour's own (ascending) face list
+            int tw = 0;
+            for (int q = off[j]; q < off[j + 1]; q++)
+                if (cf[q] == f) {
+                    tw = q - off[j];
+                    break;
+                }
+            s.nb = new_of_old[j] | (tw << SLOT_TWIN_SHIFT);
         }
         slots[(long long)k * NcPad + i] = s;
     }
+}
+
+// Pass-B face records in the new face order: A, n and the boundary face's ghost.
+__global__ void k_pack_fn(int Nf, const int* __restrict__ face_old_of_new, const FaceRec64* __restrict__ rec_new,
+                          const int2* __restrict__ fc, const int* __restrict__ bcid,
+                          const int* __restrict__ new_of_old, FaceN64* __restrict__ fn) {
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= Nf) return;
+    const int o = face_old_of_new[g];
+    const FaceRec64 r = rec_new[g];
+    FaceN64 x;
+    x.A = r.A;
+    x.n0 = r.n0;
+    x.n1 = r.n1;
+    x.ghost = bcid[o] >= 0 ? new_of_old[fc[o].y] : -1;
+    x.pad = 0;
+    fn[g] = x;
+}
+
+__global__ void k_pack_fn32(int Nf, const FaceN64* __restrict__ fn, FaceN32* __restrict__ fn32) {
+    int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= Nf) return;
+    const FaceN64 x = fn[g];
+    fn32[g] = FaceN32{(float)x.A, (float)x.n0, (float)x.n1, x.ghost};
 }
 
 __global__ void k_permute_d(int n, const int* __restrict__ old_of_new, const double* __restrict__ src,
@@ -567,11 +671,11 @@ __global__ void k_max_reduce(int n, const int* __restrict__ a, int* __restrict__
 }  // namespace
 
 void free_device_mesh(DeviceMesh& m) {
-    for (void* p : {(void*)m.slots, (void*)m.faces, (void*)m.iv, (void*)m.ghost_bc, (void*)m.inst_int_off,
-                    (void*)m.xf, (void*)m.xc})
+    for (void* p : {(void*)m.slots, (void*)m.faces, (void*)m.fn, (void*)m.iv, (void*)m.ghost_bc,
+                    (void*)m.inst_int_off, (void*)m.xf, (void*)m.xc})
         if (p) (void)hipFree(p);
-    m.slots = nullptr; m.faces = nullptr; m.iv = nullptr; m.ghost_bc = nullptr; m.inst_int_off = nullptr;
-    m.xf = nullptr; m.xc = nullptr;
+    m.slots = nullptr; m.faces = nullptr; m.fn = nullptr; m.iv = nullptr; m.ghost_bc = nullptr;
+    m.inst_int_off = nullptr; m.xf = nullptr; m.xc = nullptr;
 }
 
 void build_device_mesh(const BuildInput& in, DeviceMesh& out, hipStream_t st) {
@@ -653,7 +757,7 @@ void build_device_mesh(const BuildInput& in, DeviceMesh& out, hipStream_t st) {
     unsigned long long* d_key = sc.get<unsigned long long>(std::max(Nc, Nf));
     int* d_val = sc.get<int>(std::max(Nc, Nf));
     int* d_n = sc.get<int>(1);
-    if (in.renumber && Nint > 0) {
+    if (in.order == ORDER_RCM && Nint > 0) {
         int* d_level = sc.get<int>(Nc);
         int* d_claim = sc.get<int>(Nc);
         int* d_ideg = sc.get<int>(Nc);
@@ -719,6 +823,27 @@ void build_device_mesh(const BuildInput& in, DeviceMesh& out, hipStream_t st) {
         }
         HIPCHK(hipMemsetAsync(d_n, 0, sizeof(int), st));
         k_interior_keys<<<nblk(Nc), 256, 0, st>>>(Nc, d_inst_off, ninst, d_ghost, d_cmpos, d_key, d_val, d_n);
+    } else if (in.order == ORDER_SFC && Nint > 0) {
+        // bounding box of each instance's nodes
+        std::vector<double> bb(4 * (size_t)ninst);
+        for (int b = 0; b < ninst; b++) {
+            double x0 = 1e300, y0 = 1e300, x1 = -1e300, y1 = -1e300;
+            for (int i = in.inst_cell_off[b]; i < in.inst_cell_off[b + 1]; i++)
+                for (int q = in.cn_off[i]; q < in.cn_off[i + 1]; q++) {
+                    const double x = in.nodes[2 * (size_t)in.cn_con[q]], y = in.nodes[2 * (size_t)in.cn_con[q] + 1];
+                    x0 = std::min(x0, x); x1 = std::max(x1, x);
+                    y0 = std::min(y0, y); y1 = std::max(y1, y);
+                }
+            bb[4 * b] = x0;
+            bb[4 * b + 1] = y0;
+            bb[4 * b + 2] = x1 > x0 ? 1.0 / (x1 - x0) : 1.0;
+            bb[4 * b + 3] = y1 > y0 ? 1.0 / (y1 - y0) : 1.0;
+        }
+        double* d_bb = sc.get<double>(bb.size());
+        h2d(d_bb, bb.data(), bb.size(), st);
+        HIPCHK(hipMemsetAsync(d_n, 0, sizeof(int), st));
+        k_sfc_keys<<<nblk(Nc), 256, 0, st>>>(Nc, d_inst_off, ninst, d_ghost, d_xc, reinterpret_cast<double4*>(d_bb),
+                                              d_key, d_val, d_n);
     } else {
         HIPCHK(hipMemsetAsync(d_n, 0, sizeof(int), st));
         k_identity_keys<<<nblk(Nc), 256, 0, st>>>(Nc, d_ghost, d_key, d_val, d_n);
@@ -757,9 +882,12 @@ void build_device_mesh(const BuildInput& in, DeviceMesh& out, hipStream_t st) {
         }
         h2d(d_old_of_new, out.old_of_new.data(), Nc, st);
     }
+    if (Nc > SLOT_NB_MASK) fail(HCFD_ERR_ARG, "more than 2^29 cells");
     out.slots = dalloc<Slot>((size_t)out.maxdeg * out.NcPad);
     k_pack_slots<<<nblk(out.NcPad), 256, 0, st>>>(Nc, out.NcPad, out.maxdeg, d_old_of_new, d_off, d_cf, d_fc,
                                                   d_new_of_old, d_face_new_of_old, out.slots);
+    out.fn = dalloc<FaceN64>(Nf);
+    k_pack_fn<<<nblk(Nf), 256, 0, st>>>(Nf, d_val, out.faces, d_fc, d_bcid, d_new_of_old, out.fn);
     out.iv = dalloc<double>(out.NcPad);
     HIPCHK(hipMemsetAsync(out.iv, 0, sizeof(double) * out.NcPad, st));
     k_permute_d<<<nblk(Nc), 256, 0, st>>>(Nc, d_old_of_new, d_iv_old, out.iv);
@@ -813,6 +941,11 @@ hipError_t launch_geom32(int Nf, const FaceRec64* r, FaceRec32* r32, int n, cons
                          hipStream_t st) {
     if (Nf > 0) k_pack_faces32<<<nblk(Nf), 256, 0, st>>>(Nf, r, r32);
     if (n > 0) k_d2f<<<nblk(n), 256, 0, st>>>(n, iv, iv32);
+    return hipGetLastError();
+}
+
+hipError_t launch_fn32(int Nf, const FaceN64* fn, FaceN32* fn32, hipStream_t st) {
+    if (Nf > 0) k_pack_fn32<<<nblk(Nf), 256, 0, st>>>(Nf, fn, fn32);
     return hipGetLastError();
 }
 

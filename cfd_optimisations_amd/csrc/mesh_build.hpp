@@ -10,6 +10,9 @@ namespace hcfd {
 
 constexpr int MAX_DEGREE = 8;  // faces per cell supported by the kernels
 
+// Interior cell orders.
+enum { ORDER_CALLER = 0, ORDER_RCM = 1, ORDER_SFC = 2 };
+
 // Host view of the (possibly concatenated) input mesh in caller numbering.
 struct BuildInput {
     int Nc = 0, Nf = 0, Nn = 0, ninst = 1;
@@ -21,7 +24,7 @@ struct BuildInput {
     const int* bcid = nullptr;     // [Nf] BC table entry of a boundary face, -1 for interior faces
     const unsigned char* ghost = nullptr;  // [Nc]
     const int* inst_cell_off = nullptr;    // [ninst+1] caller cell ranges of the instances
-    bool renumber = true;
+    int order = 1;  // ORDER_*: interior cell order
 };
 
 // Device-resident products of the build.  Device pointers are owned by the
@@ -32,6 +35,7 @@ struct DeviceMesh {
     int bandwidth_in = 0, bandwidth_out = 0;
     Slot* slots = nullptr;          // [maxdeg][NcPad], new numbering
     FaceRec64* faces = nullptr;     // [Nf], new numbering
+    FaceN64* fn = nullptr;          // [Nf] A, n, ghost of each face (pass B)
     double* iv = nullptr;           // [NcPad], new numbering, ghosts 0
     int* ghost_bc = nullptr;        // [max(Nghost,1)] BC table entry per ghost
     int* inst_int_off = nullptr;    // [ninst+1] interior ranges, new numbering
@@ -52,5 +56,6 @@ void free_device_mesh(DeviceMesh& m);
 // FP32 copies of the face records and 1/V for the mixed path.
 hipError_t launch_geom32(int Nf, const FaceRec64* r, FaceRec32* r32, int n, const double* iv, float* iv32,
                          hipStream_t st);
+hipError_t launch_fn32(int Nf, const FaceN64* fn, FaceN32* fn32, hipStream_t st);
 
 }  // namespace hcfd

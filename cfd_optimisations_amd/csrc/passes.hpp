@@ -12,21 +12,22 @@ struct PassArgs {
     int Nc = 0;
     int NcPad = 0;         // slot table pitch
     int ninst = 1;
-    int ntiles = 0;        // ceil(Nint / cells_per_tile)
-    int tile_cells = 0;    // blockDim.x * cells per thread
+    int ntiles = 0;        // ceil(Nint / tile_cells)
+    int tile_cells = 0;    // cells per block
     int tiles_per_xcd = 0; // ceil(ntiles / 8) when xcd_remap
     int xcd_remap = 0;
     double gamma = 1.4;
-    const Slot* slots = nullptr;
-    const FaceRec64* faces = nullptr;
+    const Slot* slots = nullptr;      // face | side, neighbour | twin slot
+    const FaceRec64* faces = nullptr; // pass A geometry (FP64)
     const FaceRec32* faces32 = nullptr;
+    const FaceN64* fn = nullptr;      // pass B geometry (FP64)
+    const FaceN32* fn32 = nullptr;
     const double* iv = nullptr;
     const float* iv32 = nullptr;
     const double* U = nullptr;   // current state (interior + ghosts)
     double* Unext = nullptr;     // next state
-    double* G = nullptr;         // pass A output: limited gradient phi*G (or G in exact mode), 8 per cell
-    double* PHI = nullptr;       // exact mode: phi, 4 per cell
-    float* G32 = nullptr;        // mixed path: phi*G in FP32
+    double* FS = nullptr;        // face states, FP64: [NcPad][MAXDEG slots][4]
+    float* FS32 = nullptr;       // face states, FP32-mixed
     const int* inst_off = nullptr;      // [ninst+1] interior ranges
     double* pmin_first = nullptr;       // [ntiles] per-tile CFL partial of the tile's first instance
     double* pmin_last = nullptr;        // [ntiles] ... of its last instance
@@ -40,10 +41,10 @@ struct PassArgs {
 // Variant selectors.
 struct PassVariant {
     int precision = HCFD_FP64;  // HCFD_FP64 / HCFD_MIXED
-    bool exact = false;         // FP64 only: keep G and phi apart (bit-identical to Base)
     bool cfl = false;           // pass A computes the CFL partials
-    int width = 1;              // AoSoA width: 1, 8, 64
+    int width = 1;              // AoSoA width of U: 1, 8, 64
     int maxdeg = 3;             // slot count: 3, 4 or 8
+    int lpc = 1;                // pass A lanes per cell: 1 or 4 (one lane per conserved variable)
 };
 
 // Launch the passes with `grid` x `block` threads on `st`.
@@ -51,7 +52,6 @@ hipError_t launch_pass_a(const PassVariant& v, const PassArgs& a, int grid, int 
 hipError_t launch_pass_b(const PassVariant& v, const PassArgs& a, int grid, int block, hipStream_t st);
 // Per-instance CFL reduction: dt[b] = cfl * min(...); t[b] += dt[b]; last[b] = dt[b].
 hipError_t launch_dt_reduce(const PassArgs& a, double cfl, double* dt, double* t, double* last, hipStream_t st);
-// Width / degree supported by the compiled variants.
 bool pass_variant_supported(const PassVariant& v);
 int pass_degree_bucket(int maxdeg);
 

@@ -18,7 +18,6 @@ int pass_degree_bucket(int maxdeg) {
 bool pass_variant_supported(const PassVariant& v) {
     if (v.width != 1 && v.width != 8 && v.width != 64) return false;
     if (v.maxdeg != 3 && v.maxdeg != 4 && v.maxdeg != 8) return false;
-    if (v.precision == HCFD_MIXED && v.exact) return false;
     return v.precision == HCFD_FP64 || v.precision == HCFD_MIXED;
 }
 
@@ -27,11 +26,11 @@ hipError_t launch_pass_a(const PassVariant& v, const PassArgs& a, int grid, int 
     if (v.precision == HCFD_MIXED) return launch_pass_a_mixed(v, a, grid, block, st);
     return with_width(v.width, [&](auto W) {
         return with_degree(v.maxdeg, [&](auto MD) {
-            return with_bool(v.exact, [&](auto EX) {
-                return with_bool(v.cfl, [&](auto CF) {
-                    return with_bool(a.ninst > 1, [&](auto MU) {
-                        hipLaunchKernelGGL((k_pass_a_fp64<decltype(W)::value, decltype(MD)::value, decltype(EX)::value,
-                                                          decltype(CF)::value, decltype(MU)::value>),
+            return with_bool(v.cfl, [&](auto CF) {
+                return with_bool(a.ninst > 1, [&](auto MU) {
+                    return with_lanes(v.lpc, [&](auto LP) {
+                        hipLaunchKernelGGL((k_pass_a<double, decltype(W)::value, decltype(MD)::value,
+                                                     decltype(CF)::value, decltype(MU)::value, decltype(LP)::value>),
                                            dim3(grid), dim3(block), 0, st, a);
                         return hipGetLastError();
                     });
@@ -46,13 +45,10 @@ hipError_t launch_pass_b(const PassVariant& v, const PassArgs& a, int grid, int 
     if (v.precision == HCFD_MIXED) return launch_pass_b_mixed(v, a, grid, block, st);
     return with_width(v.width, [&](auto W) {
         return with_degree(v.maxdeg, [&](auto MD) {
-            return with_bool(v.exact, [&](auto EX) {
-                return with_bool(a.ninst > 1, [&](auto MU) {
-                    hipLaunchKernelGGL((k_pass_b_fp64<decltype(W)::value, decltype(MD)::value, decltype(EX)::value,
-                                                      decltype(MU)::value>),
-                                       dim3(grid), dim3(block), 0, st, a);
-                    return hipGetLastError();
-                });
+            return with_bool(a.ninst > 1, [&](auto MU) {
+                hipLaunchKernelGGL((k_pass_b<false, decltype(W)::value, decltype(MD)::value, decltype(MU)::value>),
+                                   dim3(grid), dim3(block), 0, st, a);
+                return hipGetLastError();
             });
         });
     });

@@ -9,10 +9,12 @@ hipError_t launch_pass_a_mixed(const PassVariant& v, const PassArgs& a, int grid
         return with_degree(v.maxdeg, [&](auto MD) {
             return with_bool(v.cfl, [&](auto CF) {
                 return with_bool(a.ninst > 1, [&](auto MU) {
-                    hipLaunchKernelGGL((k_pass_a_mixed<decltype(W)::value, decltype(MD)::value, decltype(CF)::value,
-                                                       decltype(MU)::value>),
-                                       dim3(grid), dim3(block), 0, st, a);
-                    return hipGetLastError();
+                    return with_lanes(v.lpc, [&](auto LP) {
+                        hipLaunchKernelGGL((k_pass_a<float, decltype(W)::value, decltype(MD)::value,
+                                                     decltype(CF)::value, decltype(MU)::value, decltype(LP)::value>),
+                                           dim3(grid), dim3(block), 0, st, a);
+                        return hipGetLastError();
+                    });
                 });
             });
         });
@@ -23,7 +25,7 @@ hipError_t launch_pass_b_mixed(const PassVariant& v, const PassArgs& a, int grid
     return with_width(v.width, [&](auto W) {
         return with_degree(v.maxdeg, [&](auto MD) {
             return with_bool(a.ninst > 1, [&](auto MU) {
-                hipLaunchKernelGGL((k_pass_b_mixed<decltype(W)::value, decltype(MD)::value, decltype(MU)::value>),
+                hipLaunchKernelGGL((k_pass_b<true, decltype(W)::value, decltype(MD)::value, decltype(MU)::value>),
                                    dim3(grid), dim3(block), 0, st, a);
                 return hipGetLastError();
             });

@@ -21,7 +21,7 @@ __global__ void k_stage_gradient(StageArgs a) {
         if (s.f < 0) break;
         const bool isR = (s.f & SLOT_SIDE_R) != 0;
         const FaceRec64 fr = a.faces[s.f & (SLOT_SIDE_R - 1)];
-        const int j = s.nb;
+        const int j = slot_nb(s);
         const double If = fr.If, omI = 1 - If;
         for (int v = 0; v < NV; v++) {
             const double ui = a.U[v * P + i], uj = a.U[v * P + j];
@@ -55,7 +55,7 @@ __global__ void k_stage_limiter(StageArgs a) {
         for (int k = 0; k < a.maxdeg; k++) {
             const Slot s = a.slots[k * P + i];
             if (s.f < 0) break;
-            const double uj = a.U[v * P + s.nb];
+            const double uj = a.U[v * P + slot_nb(s)];
             umin = fmin(umin, uj);
             umax = fmax(umax, uj);
         }
@@ -79,8 +79,9 @@ __global__ void k_stage_limiter(StageArgs a) {
 __global__ void k_stage_bc_copy(StageArgs a, int grads) {
     const int g = a.Nint + blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= a.Nc) return;
+    if (a.only_bc >= 0 && a.ghost_bc[g - a.Nint] != a.only_bc) return;
     const long long P = a.NcPad;
-    const int l = a.slots[g].nb;
+    const int l = slot_nb(a.slots[g]);
     if (grads) {
         for (int q = 0; q < 2 * NV; q++) a.G[q * P + g] = -a.G[q * P + l];
     } else {
@@ -100,7 +101,7 @@ __global__ void k_stage_flux(StageArgs a) {
         if (s.f < 0) break;
         const bool isR = (s.f & SLOT_SIDE_R) != 0;
         const FaceRec64 fr = a.faces[s.f & (SLOT_SIDE_R - 1)];
-        const int L = isR ? s.nb : i, Rc = isR ? i : s.nb;
+        const int L = isR ? slot_nb(s) : i, Rc = isR ? i : slot_nb(s);
         double UfL[NV], UfR[NV];
         for (int v = 0; v < NV; v++) {
             UfL[v] = a.U[v * P + L] + a.PHI[v * P + L] * (a.G[(2 * v) * P + L] * fr.rL0 + a.G[(2 * v + 1) * P + L] * fr.rL1);
@@ -132,11 +133,12 @@ __global__ void k_stage_update(StageArgs a, double dt) {
 __global__ void k_stage_bc_cells(StageArgs a) {
     const int g = a.Nint + blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= a.Nc) return;
+    if (a.only_bc >= 0 && a.ghost_bc[g - a.Nint] != a.only_bc) return;
     const long long P = a.NcPad;
     const Slot s = a.slots[g];
     const FaceRec64 fr = a.faces[s.f & (SLOT_SIDE_R - 1)];
     double ul[NV], ug[NV];
-    for (int v = 0; v < NV; v++) ul[v] = a.U[v * P + s.nb];
+    for (int v = 0; v < NV; v++) ul[v] = a.U[v * P + slot_nb(s)];
     ghost_state(a.bctab[a.ghost_bc[g - a.Nint]], a.gamma, ul, fr.n0, fr.n1, ug);
     for (int v = 0; v < NV; v++) a.U[v * P + g] = ug[v];
 }
@@ -231,9 +233,20 @@ __global__ void k_bc_cells_state(int Nint, int Nc, long long NcPad, const Slot* 
     const Slot s = slots[g];
     const FaceRec64 fr = faces[s.f & (SLOT_SIDE_R - 1)];
     double ul[NV], ug[NV];
-    load_cell<W, NV>(U, s.nb, ul);
+    load_cell<W, NV>(U, slot_nb(s), ul);
     ghost_state(bctab[ghost_bc[g - Nint]], gma, ul, fr.n0, fr.n1, ug);
     store_cell<W, NV>(U, g, ug);
+}
+
+// Plain streaming copy, one 16-byte element per lane (one pass over the grid).
+__global__ void k_copy16(long long n16, const double2* __restrict__ src, double2* __restrict__ dst) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n16) dst[i] = src[i];
+}
+
+hipError_t launch_copy16(long long n16, const void* src, void* dst, int grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, st, n16, (const double2*)src, (double2*)dst);
+    return hipGetLastError();
 }
 
 // ---- launchers ------------------------------------------------------------------------

@@ -28,6 +28,7 @@ struct StageArgs {
     double* G = nullptr;    // [4][2][NcPad]
     double* PHI = nullptr;  // [4][NcPad]
     double* R = nullptr;    // [4][NcPad]
+    int only_bc = -1;       // BC stages: restrict to one BC table entry (-1 = all)
 };
 
 hipError_t stage_launch(int which, const StageArgs& a, double dt, hipStream_t st);
@@ -46,6 +47,7 @@ hipError_t launch_bc_cells_state(int width, int Nint, int Nc, long long NcPad, c
                                  const FaceRec64* faces, const BCEntry* bctab, const int* ghost_bc, double gma,
                                  double* U, hipStream_t st);
 
+hipError_t launch_copy16(long long n16, const void* src, void* dst, int grid, hipStream_t st);
 hipError_t launch_soa_gather(int n, int ncomp, const int* old_of_new, const double* src, long long spitch, double* dst,
                              long long dpitch, hipStream_t st);
 hipError_t launch_soa_scatter(int n, int ncomp, const int* old_of_new, const double* src, long long spitch, double* dst,

@@ -1,0 +1,52 @@
+// MI355X::Euler -- the shape of includes/Base/TimeIntegrators.hpp:11-45.  The
+// state lives on the GPU; run() drives the fused device step between outputs.
+#pragma once
+#include <string>
+
+#include "Global.hpp"
+#include "MI355X/BoundaryConditions.hpp"
+#include "MI355X/Domain.hpp"
+#include "MI355X/Flux.hpp"
+#include "MI355X/GradientSchemes.hpp"
+#include "MI355X/Limiters.hpp"
+#include "Writer.hpp"
+
+namespace MI355X {
+
+class Euler {
+   public:
+    Euler(int nvars, Domain& domain, const Flux& flux, const GradientScheme& grads, Limiter& limiter);
+    ~Euler();
+    Euler(const Euler&) = delete;
+    Euler& operator=(const Euler&) = delete;
+    void gather_BCs(const BCtype* BCmap, const double QBC[][NVARS]);
+    void initialise(const double QIC[NVARS]);
+
+    void run(double T, double dt, double dt_out);
+    void compute_prims();
+
+    // Output file stem (the reference writes ../sols/<Layout>, src/Base/Euler.cpp:22).
+    void set_output(const std::string& fname);
+    // Conserved state copied back from the GPU into U.
+    const double* const* state();
+
+   private:
+    const int nvars, Nc, Nf;
+    Domain& domain;
+    const Flux& flux;
+    const GradientScheme& grads;
+    Limiter& limiter;
+
+    const int NBC;
+    BoundaryCondition** BCs;
+
+    double* Q[NVARS];
+    double* U[NVARS];
+    double* R[NVARS];
+    double* phi[NVARS];
+    double* G[NVARS];
+
+    Writer* writer;
+};
+
+}  // namespace MI355X

@@ -18,7 +18,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libhcfd.so")
 
 FP64, MIXED = 0, 1
-FLAG_EXACT, FLAG_NO_RENUMBER, FLAG_GRAPH = 1, 2, 4
+FLAG_EXACT, FLAG_NO_RENUMBER, FLAG_GRAPH, FLAG_ORDER_SFC = 1, 2, 4, 8
 SYMMETRY, WALL, SUPINLET, SUBINLET, SUPOUTLET, SUBOUTLET = range(6)
 ARR_U, ARR_G, ARR_PHI, ARR_R = range(4)
 
@@ -46,7 +46,7 @@ class Case(ctypes.Structure):
 
 
 class Schedule(ctypes.Structure):
-    _fields_ = [("block", I), ("cells_per_thread", I), ("aosoa_width", I), ("persistent", I), ("xcd_remap", I)]
+    _fields_ = [("block", I), ("lanes_per_cell", I), ("aosoa_width", I), ("persistent", I), ("xcd_remap", I)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -117,6 +117,9 @@ def load() -> ctypes.CDLL:
         "hcfd_meshgen_ramp": [I, I, D, D, D, D, D, ctypes.c_ulonglong, I, P],
         "hcfd_write_hsln": [ctypes.c_char_p, I, P],
         "hcfd_mesh_import_msh": [ctypes.c_char_p, P],
+        "hcfd_stage_bc": [P, I, I],
+        "hcfd_set_gamma": [P, D],
+        "hcfd_copy_bandwidth": [ctypes.c_longlong, I, P],
     }
     for name, args in sig.items():
         fn = getattr(lib, name, None)
@@ -147,6 +150,13 @@ def device_count() -> int:
     n = I(0)
     check(load().hcfd_device_count(ctypes.byref(n)))
     return n.value
+
+
+def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 20) -> float:
+    """GB/s of the library's plain copy kernel (read + write bytes / time)."""
+    g = D(0.0)
+    check(load().hcfd_copy_bandwidth(nbytes, reps, ctypes.byref(g)))
+    return g.value
 
 
 # ---------------------------------------------------------------------------
@@ -412,8 +422,8 @@ class Context:
         check(load().hcfd_get_schedule(self.h, ctypes.byref(s)))
         return s.as_dict()
 
-    def set_schedule(self, block=256, cells_per_thread=1, aosoa_width=1, persistent=0, xcd_remap=1):
-        s = Schedule(block, cells_per_thread, aosoa_width, persistent, xcd_remap)
+    def set_schedule(self, block=256, lanes_per_cell=1, aosoa_width=1, persistent=0, xcd_remap=1):
+        s = Schedule(block, lanes_per_cell, aosoa_width, persistent, xcd_remap)
         check(load().hcfd_set_schedule(self.h, ctypes.byref(s)))
 
     def autotune(self, steps: int = 10, cfl: float = 0.0, cache: Optional[str] = None) -> dict:

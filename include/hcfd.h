@@ -61,9 +61,11 @@ enum hcfd_bctype {
 /* Creation flags. */
 enum hcfd_flags {
     HCFD_FLAG_NONE = 0,
-    HCFD_FLAG_EXACT = 1,       /* FP64: keep G and phi separately so results are bit-identical to Base */
+    HCFD_FLAG_EXACT = 1,       /* accepted for compatibility: the FP64 path is always bit-identical to Base */
     HCFD_FLAG_NO_RENUMBER = 2, /* keep the caller's cell order (no RCM); faces still grouped by cell */
-    HCFD_FLAG_GRAPH = 4        /* replay multi-step runs from a captured hipGraph */
+    HCFD_FLAG_GRAPH = 4,       /* replay multi-step runs from a captured hipGraph */
+    HCFD_FLAG_ORDER_SFC = 8    /* order interior cells along a Hilbert curve of their centroids
+                                  (compact blocks) instead of reverse Cuthill-McKee */
 };
 
 /* Raw mesh arrays, exactly the members the reference's Mesh reads from a
@@ -90,7 +92,7 @@ typedef struct hcfd_case {
 /* Launch schedule of the two fused passes; filled by the autotuner. */
 typedef struct hcfd_schedule {
     int block;           /* threads per block: 64, 128, 256 */
-    int cells_per_thread;/* 1 or 2 */
+    int lanes_per_cell;  /* pass A lanes per cell: 1, or 4 (one per conserved variable; 16 cells per wavefront) */
     int aosoa_width;     /* cell-state tile width: 1 (AoS), 8, 64 */
     int persistent;      /* 0: one tile per block; >0: blocks per CU of a persistent grid */
     int xcd_remap;       /* 1: map block ids so each XCD walks a contiguous cell range */
@@ -173,6 +175,13 @@ int hcfd_stage_update(hcfd_ctx* ctx, double dt);
 /* BoundaryCondition::update_cells for every BC (Euler::update_BC_cells,
  * src/Base/Euler.cpp:138-142). */
 int hcfd_stage_bc_cells(hcfd_ctx* ctx);
+/* One BoundaryCondition object's update_cells / update_grads / update_limiter
+ * (src/Base/BoundaryConditions/*.cpp) on its own faces: BC ordinal bc (-1 = all).
+ * what: 0 = cells, 1 = gradients, 2 = limiter. */
+int hcfd_stage_bc(hcfd_ctx* ctx, int what, int bc);
+/* Ratio of specific heats used by the stage flux (Flux::Flux(nvars, gma, ...),
+ * includes/Base/Flux.hpp:9) before any BCs are set. */
+int hcfd_set_gamma(hcfd_ctx* ctx, double gamma);
 
 /* ---- fused stepping (the hot path) --------------------------------------- */
 
@@ -192,6 +201,11 @@ int hcfd_get_fault(hcfd_ctx* ctx, int* fault);
  * ms_out[2] = mean pass B (flux + update + ghost BCs); ms_out[3] = mean CFL
  * reduction (0 for fixed dt). */
 int hcfd_time_steps(hcfd_ctx* ctx, int nsteps, double dt, double cfl, float* ms_out);
+
+/* Device copy ceiling: a plain 16-byte-per-lane copy kernel over `bytes` bytes
+ * (read + write), run reps times on a private stream; *gbs = 2*bytes/time.
+ * Also the known-size kernel for calibrating HBM byte counters. */
+int hcfd_copy_bandwidth(long long bytes, int reps, double* gbs);
 
 /* Sets the launch schedule; NULL restores the default. */
 int hcfd_set_schedule(hcfd_ctx* ctx, const hcfd_schedule* s);

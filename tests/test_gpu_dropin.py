@@ -1,0 +1,53 @@
+"""The C++ drop-in classes (namespace MI355X) run the reference's own case
+(src/Test.cpp:50-74) through the C ABI; their .hsln output must match the
+oracle bit for bit and the shipped golden file to FMA noise."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from cfd_optimisations_amd import cases as C
+from cfd_optimisations_amd.formats import read_hsln
+import oracle_lib as ol
+from conftest import GOLDEN, ROOT
+
+pytestmark = pytest.mark.gpu
+
+RUN = os.path.join(ROOT, "cfd_optimisations_amd", "hcfd_run")
+
+
+def oracle_prims(wedge_mesh, steps):
+    case = C.reference_wedge_case(wedge_mesh)
+    od = ol.OracleDomain(wedge_mesh, case.gamma, case.bctypes, case.qbc)
+    od.zero_bc_iv()
+    U0 = od.initialise(case.qic)
+    U, _ = od.run(U0, steps, dt=1e-3)
+    return od.prims(U0), od.prims(U)
+
+
+@pytest.mark.parametrize("mode", ["fused", "stages"])
+def test_dropin_driver_matches_oracle(tmp_path, wedge_mesh, mode):
+    steps = 1000 if mode == "fused" else 100
+    stem = str(tmp_path / "MI355X")
+    args = [RUN, os.path.join(GOLDEN, "wedge.hmsh"), stem, str(steps)] + (["--stages"] if mode == "stages" else [])
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "MI355X" in r.stdout
+    Q0, Q1 = oracle_prims(wedge_mesh, steps)
+    _, S1 = read_hsln(stem + "_1.hsln")
+    assert np.array_equal(S1, Q1)
+    if mode == "fused":
+        _, S0 = read_hsln(stem + "_0.hsln")
+        assert np.array_equal(S0, Q0)
+        _, B1 = read_hsln(os.path.join(GOLDEN, "Base_1.hsln"))
+        err = [np.linalg.norm(S1[v] - B1[v]) / np.linalg.norm(B1[v]) for v in range(4)]
+        assert max(err) < 1e-13
+        assert os.path.getsize(stem + "_1.hsln") == os.path.getsize(os.path.join(GOLDEN, "Base_1.hsln"))
+
+
+def test_dropin_driver_reports_bad_mesh(tmp_path):
+    bad = tmp_path / "bad.hmsh"
+    bad.write_bytes(b"\x02\x00\x00\x00" * 3)
+    r = subprocess.run([RUN, str(bad), str(tmp_path / "x"), "1"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "truncated" in r.stderr
