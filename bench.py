@@ -28,10 +28,16 @@ METRIC = "cell-updates/s per residual eval + % HBM peak; FP32-mixed vs FP64 at s
 
 
 def alg_bytes(Nc: int, Nf: int, mixed: bool):
-    """Algorithmic HBM bytes of the minimal two-pass dataflow (cells incl. ghosts).
-    FP64:  pass A 104 Nc + 64 Nf (U, 1/V read; phi*G written; face geometry)
-           pass B 136 Nc + 64 Nf (U, phi*G, 1/V read; U written; face geometry)
-    mixed: pass A  68 Nc + 36 Nf;  pass B 100 Nc + 36 Nf."""
+    """Algorithmic HBM bytes per step of the minimal two-pass dataflow that the
+    method's dependencies force (cells incl. ghosts, every array touched once per
+    pass, face geometry read once per pass):
+    FP64:  pass A 104 Nc + 64 Nf (U 32 + 1/V 8 read, limited gradient 64 written;
+                                  face record 64)
+           pass B 136 Nc + 64 Nf (U 32 + limited gradient 64 + 1/V 8 read, U 32
+                                  written; face record 64)
+    mixed: pass A  68 Nc + 36 Nf;  pass B 100 Nc + 36 Nf.
+    The kernels' own intermediate (face states instead of the limited gradient)
+    is a design choice; the figure stays the method's minimum."""
     if mixed:
         a, b = 68 * Nc + 36 * Nf, 100 * Nc + 36 * Nf
     else:
@@ -39,10 +45,14 @@ def alg_bytes(Nc: int, Nf: int, mixed: bool):
     return a, b, a + b
 
 
+PMC_SUMMARY = {"c2": "pmc_riemann_4m_fp64.md", "c3": "pmc_ramp_16m_fp64.md", "c3mixed": "pmc_ramp_16m_mixed.md"}
+
+
 def read_traffic(workload: str):
-    """HBM bytes per launch of the dominant kernel from the committed rocprofv3
-    --pmc summary (profiles/pmc_<workload>.md, line 'traffic_bytes <kernel> <bytes>')."""
-    path = os.path.join(ROOT, "profiles", f"pmc_{workload}.md")
+    """HBM bytes per launch of each pass from the committed rocprofv3 --pmc summary
+    of the workload (profiles/pmc_*.md, lines 'traffic_bytes <kernel> <bytes>':
+    FETCH_SIZE and WRITE_SIZE scaled by what they read for a copy of known size)."""
+    path = os.path.join(ROOT, "profiles", PMC_SUMMARY.get(workload, "none"))
     if not os.path.exists(path):
         return None
     out = {}
@@ -55,7 +65,8 @@ def read_traffic(workload: str):
 
 def cpu_baseline(case, ctx, dt: float, target_s: float):
     """The reference's Base solver (oracle/_ref, built from its own sources) on one
-    core, on the same mesh in the GPU's RCM cell order, for a bounded number of steps."""
+    core, on the same mesh in the GPU's interior cell order, for a bounded number of
+    steps of the dt the GPU run ended with."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import numpy as np
     import oracle_lib as ol
@@ -96,7 +107,7 @@ def cpu_baseline(case, ctx, dt: float, target_s: float):
         pass
     return {"value": nint * n / sec, "unit": "cell-updates/s", "cores": 1, "kind": kind,
             "sample": f"{'reference Base (oracle/_ref)' if kind == 'reference' else 'oracle port'}, same mesh in "
-                      f"RCM order, {n} steps of fixed dt after 1 warm-up step, 1 thread on {cpu}",
+                      f"GPU's cell order, {n} steps of fixed dt after 1 warm-up step, 1 thread on {cpu}",
             "seconds": round(sec, 3)}
 
 
@@ -108,7 +119,8 @@ def build_case(name: str):
         w = W["c2_riemann_fp64"]
         return C.riemann_case(w["nx"], w["ny"], seed=0), w, L.FP64, \
             f"c2: four-quadrant Riemann problem on [0,1]^2, {w['nx']}x{w['ny']} quads -> {2 * w['nx'] * w['ny']:,} " \
-            f"triangles, nodes jittered +-20% (seed 0), cells shuffled then RCM-renumbered, FP64, CFL 0.5"
+            f"triangles, nodes jittered +-20% (seed 0), cells shuffled then renumbered on the device, FP64, " \
+            f"CFL {w['cfl']}"
     if name in ("c3", "c3mixed"):
         w = W["c3_ramp_16m"]
         prec = L.MIXED if name == "c3mixed" else L.FP64
