@@ -18,7 +18,8 @@
  *     Gradients are `X[v][d*Nc + i]` (includes/Base/GradientSchemes.hpp).
  *   - A context owns its device buffers and one HIP stream; it may be used by
  *     one host thread at a time.  Cells are renumbered internally (reverse
- *     Cuthill-McKee); every getter/setter speaks the caller's order.
+ *     Cuthill-McKee, or a Hilbert curve with HCFD_FLAG_ORDER_SFC); every
+ *     getter/setter speaks the caller's order.
  *   - No C++ exceptions cross this boundary.
  */
 #ifndef HCFD_H
