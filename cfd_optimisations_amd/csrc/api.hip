@@ -84,10 +84,9 @@ struct hcfd_ctx {
     int* d_ghost_inst = nullptr;
     double* U[2] = {nullptr, nullptr};
     int cur = 0;
-    double* FS = nullptr;    // FP64 face states [NcPad][deg_bucket][4]
-    float* FS32 = nullptr;   // FP32-mixed face states
-    FaceN32* fn32 = nullptr;
-    FaceRec32* faces32 = nullptr;
+    double* FS = nullptr;    // FP64 face states, [deg_bucket][2][NcPad] double2 planes
+    float* FS32 = nullptr;   // FP32-mixed face states, [deg_bucket][NcPad] float4
+    float* cgeo32 = nullptr; // FP32 per-slot geometry planes (mixed path)
     float* iv32 = nullptr;
     double* pmin_first = nullptr;
     double* pmin_last = nullptr;
@@ -241,8 +240,7 @@ hcfd_ctx* create_ctx(int n, const hcfd_mesh_view* views, int precision, int flag
     c->Nghost = c->m.Nghost;
     c->NcPad = c->m.NcPad;
     c->maxdeg = c->m.maxdeg;
-    c->deg_bucket = pass_degree_bucket(c->maxdeg);
-    if (c->deg_bucket < 0) fail(HCFD_ERR_MESH, "cells with more than 8 faces are not supported");
+    c->deg_bucket = c->m.nslot;
     for (int k = 0; k < n; k++)
         if (c->m.inst_int_off_h[k + 1] == c->m.inst_int_off_h[k]) fail(HCFD_ERR_MESH, "instance without interior cells");
 
@@ -265,18 +263,18 @@ hcfd_ctx* create_ctx(int n, const hcfd_mesh_view* views, int precision, int flag
     c->U[1] = dmalloc<double>(4 * P);
     HIPCHK(hipMemsetAsync(c->U[0], 0, sizeof(double) * 4 * P, c->st));
     HIPCHK(hipMemsetAsync(c->U[1], 0, sizeof(double) * 4 * P, c->st));
-    // face states, cell-major: [NcPad][deg_bucket slots][4]
+    // face states: deg_bucket slots x 4 values per cell, slot-major planes
     const size_t nfs = P * (size_t)c->deg_bucket * NV;
     if (precision == HCFD_FP64) {
         c->FS = dmalloc<double>(nfs);
     } else {
         c->FS32 = dmalloc<float>(nfs);
-        c->faces32 = dmalloc<FaceRec32>(c->Nf);
-        c->fn32 = dmalloc<FaceN32>(c->Nf);
+        const size_t ngeo = (size_t)GEO_PLANES * c->deg_bucket * P;
+        c->cgeo32 = dmalloc<float>(ngeo);
         c->iv32 = dmalloc<float>(P);
-        // FP32 geometry from the FP64 records (same values rounded)
-        HIPCHK(launch_geom32(c->Nf, c->m.faces, c->faces32, (int)P, c->m.iv, c->iv32, c->st));
-        HIPCHK(launch_fn32(c->Nf, c->m.fn, c->fn32, c->st));
+        // FP32 geometry from the FP64 planes (same values rounded)
+        HIPCHK(launch_d2f((long long)ngeo, c->m.cgeo, c->cgeo32, c->st));
+        HIPCHK(launch_d2f((long long)P, c->m.iv, c->iv32, c->st));
     }
     const int maxtiles = (c->Nint + 15) / 16 + 1;  // smallest pass-A tile: 64 threads / 4 lanes per cell
     c->pmin_first = dmalloc<double>(maxtiles);
@@ -384,11 +382,10 @@ Launch make_launch(hcfd_ctx* c, bool cfl, int cur, int pass) {
     a.xcd_remap = s.xcd_remap;
     a.tiles_per_xcd = (a.ntiles + NXCD - 1) / NXCD;
     a.gamma = c->gamma;
-    a.slots = c->m.slots;
-    a.faces = c->m.faces;
-    a.faces32 = c->faces32;
-    a.fn = c->m.fn;
-    a.fn32 = c->fn32;
+    a.cnb = c->m.cnb;
+    a.cside = c->m.cside;
+    a.cgeo = c->m.cgeo;
+    a.cgeo32 = c->cgeo32;
     a.iv = c->m.iv;
     a.iv32 = c->iv32;
     a.U = c->U[cur];
@@ -560,7 +557,7 @@ hcfd_ctx::~hcfd_ctx() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
     free_device_mesh(m);
     for (void* p : {(void*)bctab, (void*)d_old_of_new, (void*)d_ghost_inst, (void*)U[0], (void*)U[1], (void*)FS,
-                    (void*)FS32, (void*)fn32, (void*)faces32, (void*)iv32, (void*)pmin_first, (void*)pmin_last,
+                    (void*)FS32, (void*)cgeo32, (void*)iv32, (void*)pmin_first, (void*)pmin_last,
                     (void*)dtacc, (void*)dt, (void*)tdev, (void*)lastdt, (void*)fault, (void*)staging, (void*)sU,
                     (void*)sG, (void*)sPhi, (void*)sR, (void*)backup})
         free_ptr(p);
@@ -641,11 +638,11 @@ int hcfd_get_info(hcfd_ctx* c, hcfd_info* o) {
         o->ncolours = c->m.ncolours;
         size_t P = c->NcPad;
         long long b = 0;
-        b += (long long)c->maxdeg * P * sizeof(Slot) + P * 8;
-        b += (long long)c->Nf * (sizeof(FaceRec64) + sizeof(FaceN64));
+        b += (long long)c->deg_bucket * P * (sizeof(Slot) + sizeof(int) + 8 * GEO_PLANES) + P * (8 + 4);
+        b += (long long)c->Nf * sizeof(FaceRec64);
         b += 8LL * P * 8;                         // U x2
         b += (c->FS ? 32LL : c->FS32 ? 16LL : 0) * (long long)P * c->deg_bucket;
-        b += (c->faces32 ? (long long)c->Nf * (sizeof(FaceRec32) + sizeof(FaceN32)) + P * 4 : 0);
+        b += (c->cgeo32 ? 4LL * GEO_PLANES * c->deg_bucket * P + P * 4 : 0);
         b += 8LL * c->Nc * 8 + 16LL * c->Nf + 16LL * c->Nc;  // staging, xf, xc
         o->device_bytes = b;
         o->sched = c->sched;

@@ -34,26 +34,12 @@ struct alignas(64) FaceRec64 {
 };
 static_assert(sizeof(FaceRec64) == 64, "FaceRec64 must be 64 bytes");
 
-// Face geometry record, FP32-mixed path (32 bytes).
-struct alignas(32) FaceRec32 {
-    float A, n0, n1, rL0, rL1, rR0, rR1, If;
-};
-static_assert(sizeof(FaceRec32) == 32, "FaceRec32 must be 32 bytes");
-
-// Flux geometry of a face for pass B, FP64 (32 bytes): A, unit normal L->R,
-// and the ghost cell (new numbering) of a boundary face, else -1.
-struct alignas(32) FaceN64 {
-    double A, n0, n1;
-    int ghost, pad;
-};
-static_assert(sizeof(FaceN64) == 32, "FaceN64 must be 32 bytes");
-
-// Same for the FP32-mixed path (16 bytes).
-struct alignas(16) FaceN32 {
-    float A, n0, n1;
-    int ghost;
-};
-static_assert(sizeof(FaceN32) == 16, "FaceN32 must be 16 bytes");
+// Per-slot geometry planes of the fused passes (cell-major, one plane per
+// quantity and slot): plane p of slot k of cell i lives at
+// (p * nslot + k) * NcPad + i.  n is the face normal oriented OUT of cell i
+// (the L->R normal, negated when i is the face's R cell); r is the face centre
+// minus the centre of cell i (rL or rR); If is the face's weight of U_L.
+enum { GEO_A = 0, GEO_N0, GEO_N1, GEO_IF, GEO_R0, GEO_R1, GEO_PLANES };
 
 // Boundary-condition table entry (one per BC ordinal per instance).
 struct BCEntry {
@@ -71,6 +57,15 @@ struct Slot {
 };
 constexpr int SLOT_TWIN_SHIFT = 29;
 constexpr int SLOT_NB_MASK = (1 << SLOT_TWIN_SHIFT) - 1;
+
+// Slot count the kernels are compiled for, from the largest cell degree:
+// 3 (triangles), 4 (quadrilaterals), 8 (polygons up to octagons); -1 above.
+inline int degree_bucket(int maxdeg) {
+    if (maxdeg <= 3) return 3;
+    if (maxdeg <= 4) return 4;
+    if (maxdeg <= 8) return 8;
+    return -1;
+}
 
 inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 inline long long round_up_ll(long long x, long long m) { return (x + m - 1) / m * m; }

@@ -21,11 +21,11 @@ __device__ __forceinline__ T dmax(T a, T b) { return fmax(a, b); }
 __device__ __forceinline__ float dmin(float a, float b) { return fminf(a, b); }
 __device__ __forceinline__ float dmax(float a, float b) { return fmaxf(a, b); }
 
-// Flux::EulerFlux, src/Base/Flux.cpp:89-97.
+// Flux::EulerFlux, src/Base/Flux.cpp:89-97, given ir = 1 / U[0] (the value
+// Rusanov already holds for the same state).
 template <typename T>
-__device__ __forceinline__ void euler_flux(T gma, const T U[NV], T n0, T n1, T F[NV]) {
+__device__ __forceinline__ void euler_flux(T gma, const T U[NV], T ir, T n0, T n1, T F[NV]) {
     const T gmo = gma - T(1);
-    const T ir = T(1) / U[0];
     const T P = gmo * (U[3] - T(0.5) * ir * (U[1] * U[1] + U[2] * U[2]));
     F[0] = n0 * U[1] + n1 * U[2];
     F[1] = n0 * (U[1] * U[1] * ir + P) + n1 * U[2] * U[1] * ir;
@@ -51,20 +51,23 @@ __device__ __forceinline__ void rusanov(T gma, const T UL[NV], const T UR[NV], T
     const T S2 = dmax(fabs(nvl + al), fabs(nvr + ar));
     const T S = dmax(S1, S2);
     T FL[NV], FR[NV];
-    euler_flux(gma, UL, n0, n1, FL);
-    euler_flux(gma, UR, n0, n1, FR);
+    euler_flux(gma, UL, irl, n0, n1, FL);
+    euler_flux(gma, UR, irr, n0, n1, FR);
 #pragma unroll
     for (int v = 0; v < NV; v++) F[v] = T(0.5) * ((FL[v] + FR[v]) - S * (UR[v] - UL[v]));
 }
 
-// Limiter face candidate, src/Base/Limiter.cpp:262-266 (eps = 1e-9, :20).
+// Limiter face candidate, src/Base/Limiter.cpp:262-266 (eps = 1e-9, :20):
+// min(1, (Umax-U)/d) if d > eps, min(1, (Umin-U)/d) if d < -eps, else 1.
+// The numerator is selected before the one division, so each face costs one
+// divide instead of two; the quotient of the taken branch is the same value.
 template <typename T>
 __device__ __forceinline__ T limiter_face(T d, T umin_minus_u, T umax_minus_u) {
     const T eps = T(1e-9);
-    T p = T(1);
-    p = (d > eps) ? dmin(T(1), umax_minus_u / d) : p;
-    p = (d < -eps) ? dmin(T(1), umin_minus_u / d) : p;
-    return p;
+    const bool up = d > eps;
+    const bool take = up || (d < -eps);
+    const T q = (up ? umax_minus_u : umin_minus_u) / d;
+    return take ? dmin(T(1), q) : T(1);
 }
 
 // Cell-state access with an AoSoA tile of width W: component c of cell i of an

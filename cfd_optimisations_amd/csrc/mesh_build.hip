@@ -420,17 +420,6 @@ __global__ void k_pack_faces(int Nf, const int* __restrict__ face_old_of_new, co
     rec_new[g] = rec_old[face_old_of_new[g]];
 }
 
-__global__ void k_pack_faces32(int Nf, const FaceRec64* __restrict__ rec, FaceRec32* __restrict__ rec32) {
-    int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= Nf) return;
-    FaceRec64 r = rec[g];
-    FaceRec32 q;
-    q.A = (float)r.A; q.n0 = (float)r.n0; q.n1 = (float)r.n1;
-    q.rL0 = (float)r.rL0; q.rL1 = (float)r.rL1; q.rR0 = (float)r.rR0; q.rR1 = (float)r.rR1;
-    q.If = (float)r.If;
-    rec32[g] = q;
-}
-
 // Slots of every cell (new numbering), kept in ascending ORIGINAL face id so
 // each cell sums its faces in the reference's order.
 __global__ void k_pack_slots(int Nc, int NcPad, int maxdeg, const int* __restrict__ old_of_new,
@@ -466,28 +455,36 @@ __global__ void k_pack_slots(int Nc, int NcPad, int maxdeg, const int* __restric
     }
 }
 
-// Pass-B face records in the new face order: A, n and the boundary face's ghost.
-__global__ void k_pack_fn(int Nf, const int* __restrict__ face_old_of_new, const FaceRec64* __restrict__ rec_new,
-                          const int2* __restrict__ fc, const int* __restrict__ bcid,
-                          const int* __restrict__ new_of_old, FaceN64* __restrict__ fn) {
-    int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= Nf) return;
-    const int o = face_old_of_new[g];
-    const FaceRec64 r = rec_new[g];
-    FaceN64 x;
-    x.A = r.A;
-    x.n0 = r.n0;
-    x.n1 = r.n1;
-    x.ghost = bcid[o] >= 0 ? new_of_old[fc[o].y] : -1;
-    x.pad = 0;
-    fn[g] = x;
-}
-
-__global__ void k_pack_fn32(int Nf, const FaceN64* __restrict__ fn, FaceN32* __restrict__ fn32) {
-    int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= Nf) return;
-    const FaceN64 x = fn[g];
-    fn32[g] = FaceN32{(float)x.A, (float)x.n0, (float)x.n1, x.ghost};
+// Per-slot cell-major geometry of the fused passes (plane layout GEO_*),
+// neighbour|twin indices and side bits, from the slot table and the face
+// records.  Slots past a cell's degree hold SLOT_EMPTY and zero geometry.
+__global__ void k_pack_cellgeo(int NcPad, int nslot, const Slot* __restrict__ slots,
+                               const FaceRec64* __restrict__ faces, int* __restrict__ cnb,
+                               int* __restrict__ cside, double* __restrict__ cgeo) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= NcPad) return;
+    const long long P = NcPad;
+    int side = 0;
+    for (int k = 0; k < nslot; k++) {
+        const Slot s = slots[k * P + i];
+        double g[GEO_PLANES] = {0, 0, 0, 0, 0, 0};
+        int nbx = SLOT_EMPTY;
+        if (s.f >= 0) {
+            const bool isR = (s.f & SLOT_SIDE_R) != 0;
+            const FaceRec64 r = faces[s.f & (SLOT_SIDE_R - 1)];
+            g[GEO_A] = r.A;
+            g[GEO_N0] = isR ? -r.n0 : r.n0;
+            g[GEO_N1] = isR ? -r.n1 : r.n1;
+            g[GEO_IF] = r.If;
+            g[GEO_R0] = isR ? r.rR0 : r.rL0;
+            g[GEO_R1] = isR ? r.rR1 : r.rL1;
+            nbx = s.nb;
+            if (isR) side |= 1 << k;
+        }
+        cnb[k * P + i] = nbx;
+        for (int p = 0; p < GEO_PLANES; p++) cgeo[((long long)p * nslot + k) * P + i] = g[p];
+    }
+    cside[i] = side;
 }
 
 __global__ void k_permute_d(int n, const int* __restrict__ old_of_new, const double* __restrict__ src,
@@ -497,8 +494,8 @@ __global__ void k_permute_d(int n, const int* __restrict__ old_of_new, const dou
     dst[i] = src[old_of_new[i]];
 }
 
-__global__ void k_d2f(int n, const double* __restrict__ src, float* __restrict__ dst) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_d2f(long long n, const double* __restrict__ src, float* __restrict__ dst) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     dst[i] = (float)src[i];
 }
@@ -671,10 +668,11 @@ __global__ void k_max_reduce(int n, const int* __restrict__ a, int* __restrict__
 }  // namespace
 
 void free_device_mesh(DeviceMesh& m) {
-    for (void* p : {(void*)m.slots, (void*)m.faces, (void*)m.fn, (void*)m.iv, (void*)m.ghost_bc,
-                    (void*)m.inst_int_off, (void*)m.xf, (void*)m.xc})
+    for (void* p : {(void*)m.slots, (void*)m.faces, (void*)m.cnb, (void*)m.cside, (void*)m.cgeo, (void*)m.iv,
+                    (void*)m.ghost_bc, (void*)m.inst_int_off, (void*)m.xf, (void*)m.xc})
         if (p) (void)hipFree(p);
-    m.slots = nullptr; m.faces = nullptr; m.fn = nullptr; m.iv = nullptr; m.ghost_bc = nullptr;
+    m.slots = nullptr; m.faces = nullptr; m.cnb = nullptr; m.cside = nullptr; m.cgeo = nullptr; m.iv = nullptr;
+    m.ghost_bc = nullptr;
     m.inst_int_off = nullptr; m.xf = nullptr; m.xc = nullptr;
 }
 
@@ -883,11 +881,16 @@ void build_device_mesh(const BuildInput& in, DeviceMesh& out, hipStream_t st) {
         h2d(d_old_of_new, out.old_of_new.data(), Nc, st);
     }
     if (Nc > SLOT_NB_MASK) fail(HCFD_ERR_ARG, "more than 2^29 cells");
-    out.slots = dalloc<Slot>((size_t)out.maxdeg * out.NcPad);
-    k_pack_slots<<<nblk(out.NcPad), 256, 0, st>>>(Nc, out.NcPad, out.maxdeg, d_old_of_new, d_off, d_cf, d_fc,
+    out.nslot = degree_bucket(out.maxdeg);
+    if (out.nslot < 0) fail(HCFD_ERR_MESH, "cells with more than 8 faces are not supported");
+    out.slots = dalloc<Slot>((size_t)out.nslot * out.NcPad);
+    k_pack_slots<<<nblk(out.NcPad), 256, 0, st>>>(Nc, out.NcPad, out.nslot, d_old_of_new, d_off, d_cf, d_fc,
                                                   d_new_of_old, d_face_new_of_old, out.slots);
-    out.fn = dalloc<FaceN64>(Nf);
-    k_pack_fn<<<nblk(Nf), 256, 0, st>>>(Nf, d_val, out.faces, d_fc, d_bcid, d_new_of_old, out.fn);
+    out.cnb = dalloc<int>((size_t)out.nslot * out.NcPad);
+    out.cside = dalloc<int>(out.NcPad);
+    out.cgeo = dalloc<double>((size_t)GEO_PLANES * out.nslot * out.NcPad);
+    k_pack_cellgeo<<<nblk(out.NcPad), 256, 0, st>>>(out.NcPad, out.nslot, out.slots, out.faces, out.cnb, out.cside,
+                                                    out.cgeo);
     out.iv = dalloc<double>(out.NcPad);
     HIPCHK(hipMemsetAsync(out.iv, 0, sizeof(double) * out.NcPad, st));
     k_permute_d<<<nblk(Nc), 256, 0, st>>>(Nc, d_old_of_new, d_iv_old, out.iv);
@@ -937,15 +940,8 @@ void build_device_mesh(const BuildInput& in, DeviceMesh& out, hipStream_t st) {
     HIPCHK(hipStreamSynchronize(st));
 }
 
-hipError_t launch_geom32(int Nf, const FaceRec64* r, FaceRec32* r32, int n, const double* iv, float* iv32,
-                         hipStream_t st) {
-    if (Nf > 0) k_pack_faces32<<<nblk(Nf), 256, 0, st>>>(Nf, r, r32);
-    if (n > 0) k_d2f<<<nblk(n), 256, 0, st>>>(n, iv, iv32);
-    return hipGetLastError();
-}
-
-hipError_t launch_fn32(int Nf, const FaceN64* fn, FaceN32* fn32, hipStream_t st) {
-    if (Nf > 0) k_pack_fn32<<<nblk(Nf), 256, 0, st>>>(Nf, fn, fn32);
+hipError_t launch_d2f(long long n, const double* src, float* dst, hipStream_t st) {
+    if (n > 0) k_d2f<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, src, dst);
     return hipGetLastError();
 }
 

@@ -31,11 +31,14 @@ struct BuildInput {
 // context that requested the build and freed with it.
 struct DeviceMesh {
     int Nc = 0, Nf = 0, Nint = 0, Nghost = 0, NcPad = 0, maxdeg = 0, ninst = 1;
+    int nslot = 3;                  // degree_bucket(maxdeg): slot planes of every per-slot array
     int ncolours = 0;
     int bandwidth_in = 0, bandwidth_out = 0;
-    Slot* slots = nullptr;          // [maxdeg][NcPad], new numbering
+    Slot* slots = nullptr;          // [nslot][NcPad], new numbering
     FaceRec64* faces = nullptr;     // [Nf], new numbering
-    FaceN64* fn = nullptr;          // [Nf] A, n, ghost of each face (pass B)
+    int* cnb = nullptr;             // [nslot][NcPad] neighbour | twin slot << 29, SLOT_EMPTY
+    int* cside = nullptr;           // [NcPad] bit k: the cell is the R side of its slot-k face
+    double* cgeo = nullptr;         // [GEO_PLANES][nslot][NcPad] per-slot geometry
     double* iv = nullptr;           // [NcPad], new numbering, ghosts 0
     int* ghost_bc = nullptr;        // [max(Nghost,1)] BC table entry per ghost
     int* inst_int_off = nullptr;    // [ninst+1] interior ranges, new numbering
@@ -53,9 +56,7 @@ struct DeviceMesh {
 // Throws hcfd::Error on invalid meshes or HIP failures.
 void build_device_mesh(const BuildInput& in, DeviceMesh& out, hipStream_t stream);
 void free_device_mesh(DeviceMesh& m);
-// FP32 copies of the face records and 1/V for the mixed path.
-hipError_t launch_geom32(int Nf, const FaceRec64* r, FaceRec32* r32, int n, const double* iv, float* iv32,
-                         hipStream_t st);
-hipError_t launch_fn32(int Nf, const FaceN64* fn, FaceN32* fn32, hipStream_t st);
+// dst[i] = (float)src[i], i < n (FP32 geometry and 1/V of the mixed path).
+hipError_t launch_d2f(long long n, const double* src, float* dst, hipStream_t st);
 
 }  // namespace hcfd

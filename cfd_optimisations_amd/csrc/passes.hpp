@@ -17,17 +17,16 @@ struct PassArgs {
     int tiles_per_xcd = 0; // ceil(ntiles / 8) when xcd_remap
     int xcd_remap = 0;
     double gamma = 1.4;
-    const Slot* slots = nullptr;      // face | side, neighbour | twin slot
-    const FaceRec64* faces = nullptr; // pass A geometry (FP64)
-    const FaceRec32* faces32 = nullptr;
-    const FaceN64* fn = nullptr;      // pass B geometry (FP64)
-    const FaceN32* fn32 = nullptr;
+    const int* cnb = nullptr;         // [MAXDEG][NcPad] neighbour | twin slot << 29, or SLOT_EMPTY
+    const int* cside = nullptr;       // [NcPad] side bits (bit k: R side of slot k's face)
+    const double* cgeo = nullptr;     // [GEO_PLANES][MAXDEG][NcPad] per-slot geometry, FP64
+    const float* cgeo32 = nullptr;    // same, FP32 (mixed path)
     const double* iv = nullptr;
     const float* iv32 = nullptr;
     const double* U = nullptr;   // current state (interior + ghosts)
     double* Unext = nullptr;     // next state
-    double* FS = nullptr;        // face states, FP64: [NcPad][MAXDEG slots][4]
-    float* FS32 = nullptr;       // face states, FP32-mixed
+    double* FS = nullptr;        // face states, FP64: [MAXDEG][2][NcPad] double2 planes
+    float* FS32 = nullptr;       // face states, FP32-mixed: [MAXDEG][NcPad] float4
     const int* inst_off = nullptr;      // [ninst+1] interior ranges
     double* pmin_first = nullptr;       // [ntiles] per-tile CFL partial of the tile's first instance
     double* pmin_last = nullptr;        // [ntiles] ... of its last instance

@@ -8,12 +8,7 @@ namespace hcfd {
 hipError_t launch_pass_a_mixed(const PassVariant& v, const PassArgs& a, int grid, int block, hipStream_t st);
 hipError_t launch_pass_b_mixed(const PassVariant& v, const PassArgs& a, int grid, int block, hipStream_t st);
 
-int pass_degree_bucket(int maxdeg) {
-    if (maxdeg <= 3) return 3;
-    if (maxdeg <= 4) return 4;
-    if (maxdeg <= 8) return 8;
-    return -1;
-}
+int pass_degree_bucket(int maxdeg) { return degree_bucket(maxdeg); }
 
 bool pass_variant_supported(const PassVariant& v) {
     if (v.width != 1 && v.width != 8 && v.width != 64) return false;

@@ -8,21 +8,34 @@
 //   pass A  per cell: Green-Gauss gradient (GaussGreen.cpp:14-40), min/max
 //           limiter (Limiter.cpp:19-77) [+ CFL bound]; then the MUSCL state of
 //           every face of the cell on the cell's own side, U + phi (G . r)
-//           (Flux.cpp:83-87), into cell-major storage FS[cell][slot] (each lane
-//           writes one contiguous run); a cell next to a boundary also writes
-//           the ghost's single slot, U_g + phi (-G . rR), since the ghost's
-//           gradient is -G_L and its limiter phi_L (BoundaryConditions/Base.cpp:29-53).
+//           (Flux.cpp:83-87), into slot-major planes FS[slot][cell]; a cell next
+//           to a boundary also writes the ghost's single slot, U_g + phi (-G . rR),
+//           since the ghost's gradient is -G_L and its limiter phi_L
+//           (BoundaryConditions/Base.cpp:29-53).
 //   pass B  per cell: Rusanov flux of each of its faces (Flux.cpp:52-109) from
-//           its own FS[cell][k] and the neighbour's FS[nb][twin] (twin = the
+//           its own FS[k][cell] and the neighbour's FS[twin][nb] (twin = the
 //           face's slot in the neighbour's list), residual * 1/V, U += dt R
 //           (Euler.cpp:107-113), and the ghost states of its boundary faces
-//           from the new state (BoundaryConditions/*.cpp).
+//           from the new state (BoundaryConditions/{Symmetry,Wall,Inlet,Outlet}.cpp).
 //
-// Each cell sums its faces in ascending original face id -- the order the
-// reference adds them -- and a face's flux is computed identically by both of
-// its cells from the same stored face states, so in FP64 every value equals the
-// reference's bit for bit (with -ffp-contract=off).
+// Geometry is read from per-slot planes (cell-major, so each load of a wave is
+// one contiguous run) with the normal oriented out of the cell.  Each cell sums
+// its faces in ascending original face id -- the order the reference adds
+// them -- and the arithmetic stays the reference's value for value:
+//   - gradient: G_L += (uf A) n and G_R -= (uf A) n equal G += (uf A) n_out,
+//     since x * (-y) = -(x y) and g - x = g + (-x) exactly; uf's two products
+//     are the same pair in either order (addition commutes exactly);
+//   - flux: Rusanov is exactly antisymmetric, F(U_R, U_L, -n) = -F(U_L, U_R, n)
+//     term by term (negation and the commutativity of +, * and max are exact),
+//     so every cell evaluates the flux in its outward frame from its own state
+//     and its neighbour's, and R -= A F_out equals the reference's R_L -= A F /
+//     R_R += A F;
+//   - the ghost side of a boundary face: rR = -rL there (Domain.cpp:84-88), so
+//     (-G) . rR = G . rL exactly.
+// With -ffp-contract=off the FP64 results equal the reference's bit for bit.
 #pragma once
+#include <type_traits>
+
 #include "device_math.hpp"
 #include "passes.hpp"
 
@@ -87,13 +100,35 @@ __device__ __forceinline__ void cfl_tile(const PassArgs& a, int tile, int i, dou
     }
 }
 
-__device__ __forceinline__ void store_fs(double* p, const double v[NV]) {
-    double2* q = reinterpret_cast<double2*>(p);
-    q[0] = make_double2(v[0], v[1]);
-    q[1] = make_double2(v[2], v[3]);
+// Face-state planes: FP64 slot k of cell i is two double2 at planes 2k, 2k+1;
+// FP32 slot k is one float4 at plane k (plane pitch NcPad).
+__device__ __forceinline__ void store_fs(double* fs, long long P, int k, int i, const double v[NV]) {
+    double2* q = reinterpret_cast<double2*>(fs);
+    q[(2 * k) * P + i] = make_double2(v[0], v[1]);
+    q[(2 * k + 1) * P + i] = make_double2(v[2], v[3]);
 }
-__device__ __forceinline__ void store_fs(float* p, const float v[NV]) {
-    *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], v[3]);
+__device__ __forceinline__ void store_fs(float* fs, long long P, int k, int i, const float v[NV]) {
+    reinterpret_cast<float4*>(fs)[k * P + i] = make_float4(v[0], v[1], v[2], v[3]);
+}
+__device__ __forceinline__ void load_fs(const double* fs, long long P, int k, int i, double v[NV]) {
+    const double2* q = reinterpret_cast<const double2*>(fs);
+    const double2 a = q[(2 * k) * P + i], b = q[(2 * k + 1) * P + i];
+    v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+}
+__device__ __forceinline__ void load_fs(const float* fs, long long P, int k, int i, float v[NV]) {
+    const float4 a = reinterpret_cast<const float4*>(fs)[k * P + i];
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+}
+// Element index of component v of face state (k, i) in the planes above.
+template <typename T>
+__device__ __forceinline__ long long fs_index(long long P, int k, int i, int v) {
+    if constexpr (sizeof(T) == 8) return (((2 * k + (v >> 1)) * P + i) << 1) + (v & 1);
+    else return ((k * P + i) << 2) + v;
+}
+// Geometry plane p of slot k of cell i (MAXDEG slot planes per quantity).
+template <int MAXDEG, typename T>
+__device__ __forceinline__ T geo(const T* __restrict__ g, long long P, int p, int k, int i) {
+    return g[(long long)(p * MAXDEG + k) * P + i];
 }
 
 // ---------------------------------------------------------------------------
@@ -104,28 +139,44 @@ template <typename T>
 struct PassATypes;
 template <>
 struct PassATypes<double> {
-    using Rec = FaceRec64;
-    __device__ static const Rec* rec(const PassArgs& a) { return a.faces; }
+    __device__ static const double* geo(const PassArgs& a) { return a.cgeo; }
     __device__ static double* fs(const PassArgs& a) { return a.FS; }
     __device__ static double iv(const PassArgs& a, int i) { return a.iv[i]; }
 };
 template <>
 struct PassATypes<float> {
-    using Rec = FaceRec32;
-    __device__ static const Rec* rec(const PassArgs& a) { return a.faces32; }
+    __device__ static const float* geo(const PassArgs& a) { return a.cgeo32; }
     __device__ static float* fs(const PassArgs& a) { return a.FS32; }
     __device__ static float iv(const PassArgs& a, int i) { return a.iv32[i]; }
 };
 
+// Tile staging: the block's cells [tb, tb + tc) have their state in LDS
+// (two double2 per cell); a neighbour inside the tile is read from there.
+__device__ __forceinline__ void lds_cell(const double2* __restrict__ s, int l, double out[NV]) {
+    const double2 x = s[2 * l], y = s[2 * l + 1];
+    out[0] = x.x; out[1] = x.y; out[2] = y.x; out[3] = y.y;
+}
+template <int W>
+__device__ __forceinline__ void tile_or_global_cell(const PassArgs& a, const double2* __restrict__ s, int tb, int tc,
+                                                    int j, double out[NV]) {
+    const unsigned l = (unsigned)(j - tb);
+    if (l < (unsigned)tc) lds_cell(s, (int)l, out);
+    else load_cell<W, NV>(a.U, j, out);
+}
+
 template <typename T, int W, int MAXDEG, bool CFL>
-__device__ __forceinline__ double pass_a_cell(const PassArgs& a, int i) {
+__device__ __forceinline__ double pass_a_cell(const PassArgs& a, int i, const double2* __restrict__ sU, int tb,
+                                              int tc) {
     using PT = PassATypes<T>;
+    const long long P = a.NcPad;
+    const T* __restrict__ gp = PT::geo(a);
     double ud[NV];
-    load_cell<W, NV>(a.U, i, ud);
+    lds_cell(sU, i - tb, ud);
     T u[NV];
 #pragma unroll
     for (int v = 0; v < NV; v++) u[v] = (T)ud[v];
     const T iv = PT::iv(a, i);
+    const int side = a.cside[i];
     T g[NV][2], umin[NV], umax[NV];
 #pragma unroll
     for (int v = 0; v < NV; v++) {
@@ -151,37 +202,34 @@ __device__ __forceinline__ double pass_a_cell(const PassArgs& a, int i) {
     for (int k = 0; k < MAXDEG; k++) {
         r0[k] = T(0);
         r1[k] = T(0);
-        const Slot s = a.slots[(long long)k * a.NcPad + i];
-        if (s.f < 0) continue;
+        const int nbx = a.cnb[k * P + i];
+        if (nbx == SLOT_EMPTY) continue;
         deg = k + 1;
-        const bool isR = (s.f & SLOT_SIDE_R) != 0;
-        const int f = s.f & (SLOT_SIDE_R - 1);
-        const typename PT::Rec fr = PT::rec(a)[f];
-        const int nb = slot_nb(s);
+        const int nb = nbx & SLOT_NB_MASK;
+        const T A = geo<MAXDEG>(gp, P, GEO_A, k, i);
+        const T n0 = geo<MAXDEG>(gp, P, GEO_N0, k, i);
+        const T n1 = geo<MAXDEG>(gp, P, GEO_N1, k, i);
+        const T If = geo<MAXDEG>(gp, P, GEO_IF, k, i);
+        r0[k] = geo<MAXDEG>(gp, P, GEO_R0, k, i);
+        r1[k] = geo<MAXDEG>(gp, P, GEO_R1, k, i);
         double und[NV];
-        load_cell<W, NV>(a.U, nb, und);
-        const T If = fr.If, omI = T(1) - If;
+        tile_or_global_cell<W>(a, sU, tb, tc, nb, und);
+        // uf = U_L If + U_R (1 - If): w_own multiplies this cell's state
+        const T omI = T(1) - If;
+        const bool isR = (side >> k) & 1;
+        const T wo = isR ? omI : If, wn = isR ? If : omI;
 #pragma unroll
         for (int v = 0; v < NV; v++) {
-            // uf = U_L If + U_R (1 - If); G_L += uf A n, G_R -= uf A n
             const T un = (T)und[v];
-            const T uf = isR ? (un * If + u[v] * omI) : (u[v] * If + un * omI);
-            const T au = uf * fr.A;
-            const T x0 = au * fr.n0, x1 = au * fr.n1;
-            if (isR) {
-                g[v][0] = g[v][0] - x0;
-                g[v][1] = g[v][1] - x1;
-            } else {
-                g[v][0] = g[v][0] + x0;
-                g[v][1] = g[v][1] + x1;
-            }
+            const T uf = u[v] * wo + un * wn;
+            const T au = uf * A;
+            g[v][0] = g[v][0] + au * n0;
+            g[v][1] = g[v][1] + au * n1;
             umin[v] = dmin(umin[v], un);
             umax[v] = dmax(umax[v], un);
         }
-        r0[k] = isR ? fr.rR0 : fr.rL0;
-        r1[k] = isR ? fr.rR1 : fr.rL1;
         if (nb >= a.Nint) ghosts |= 1u << k;
-        if constexpr (CFL) lam += (double)fr.A * (fabs(ux * (double)fr.n0 + uy * (double)fr.n1) + c);
+        if constexpr (CFL) lam += (double)A * (fabs(ux * (double)n0 + uy * (double)n1) + c);
     }
 #pragma unroll
     for (int v = 0; v < NV; v++) {
@@ -202,25 +250,26 @@ __device__ __forceinline__ double pass_a_cell(const PassArgs& a, int i) {
         }
     }
     // MUSCL face states on this cell's side, U + phi (G . r)
-    T* FS = PT::fs(a) + (long long)i * (MAXDEG * NV);
+    T* FS = PT::fs(a);
 #pragma unroll
     for (int k = 0; k < MAXDEG; k++) {
         if (k < deg) {
-            T uf[NV];
+            T d[NV], uf[NV];
 #pragma unroll
-            for (int v = 0; v < NV; v++) uf[v] = u[v] + phi[v] * (g[v][0] * r0[k] + g[v][1] * r1[k]);
-            store_fs(FS + k * NV, uf);
+            for (int v = 0; v < NV; v++) {
+                d[v] = g[v][0] * r0[k] + g[v][1] * r1[k];
+                uf[v] = u[v] + phi[v] * d[v];
+            }
+            store_fs(FS, P, k, i, uf);
             if ((ghosts >> k) & 1u) {
-                // the ghost side (R) of a boundary face: U_g + phi_L ((-G_L) . rR)
-                const Slot s = a.slots[(long long)k * a.NcPad + i];
-                const typename PT::Rec fr = PT::rec(a)[s.f & (SLOT_SIDE_R - 1)];
-                const int gc = slot_nb(s);
+                // the ghost side (R) of a boundary face: U_g + phi_L ((-G_L) . rR), rR = -rL
+                const int gc = a.cnb[k * P + i] & SLOT_NB_MASK;
                 double ugd[NV];
-                load_cell<W, NV>(a.U, gc, ugd);
+                tile_or_global_cell<W>(a, sU, tb, tc, gc, ugd);
                 T ufg[NV];
 #pragma unroll
-                for (int v = 0; v < NV; v++) ufg[v] = (T)ugd[v] + phi[v] * ((-g[v][0]) * fr.rR0 + (-g[v][1]) * fr.rR1);
-                store_fs(PT::fs(a) + (long long)gc * (MAXDEG * NV), ufg);
+                for (int v = 0; v < NV; v++) ufg[v] = (T)ugd[v] + phi[v] * d[v];
+                store_fs(FS, P, 0, gc, ufg);
             }
         }
     }
@@ -235,9 +284,12 @@ __device__ __forceinline__ double pass_a_cell(const PassArgs& a, int i) {
 template <typename T, int W, int MAXDEG, bool CFL>
 __device__ __forceinline__ double pass_a_cell_quad(const PassArgs& a, int i, int v) {
     using PT = PassATypes<T>;
+    const long long P = a.NcPad;
+    const T* __restrict__ gp = PT::geo(a);
     const double udv = a.U[cidx<W, NV>(i, v)];
     const T u = (T)udv;
     const T iv = PT::iv(a, i);
+    const int side = a.cside[i];
     T g0 = T(0), g1 = T(0), umin = u, umax = u;
     double ux = 0, uy = 0, c = 0, lam = 0.0;
     if constexpr (CFL) {
@@ -258,31 +310,27 @@ __device__ __forceinline__ double pass_a_cell_quad(const PassArgs& a, int i, int
     for (int k = 0; k < MAXDEG; k++) {
         r0[k] = T(0);
         r1[k] = T(0);
-        const Slot s = a.slots[(long long)k * a.NcPad + i];
-        if (s.f < 0) continue;
+        const int nbx = a.cnb[k * P + i];
+        if (nbx == SLOT_EMPTY) continue;
         deg = k + 1;
-        const bool isR = (s.f & SLOT_SIDE_R) != 0;
-        const int f = s.f & (SLOT_SIDE_R - 1);
-        const typename PT::Rec fr = PT::rec(a)[f];
-        const int nb = slot_nb(s);
+        const int nb = nbx & SLOT_NB_MASK;
+        const T A = geo<MAXDEG>(gp, P, GEO_A, k, i);
+        const T n0 = geo<MAXDEG>(gp, P, GEO_N0, k, i);
+        const T n1 = geo<MAXDEG>(gp, P, GEO_N1, k, i);
+        const T If = geo<MAXDEG>(gp, P, GEO_IF, k, i);
+        r0[k] = geo<MAXDEG>(gp, P, GEO_R0, k, i);
+        r1[k] = geo<MAXDEG>(gp, P, GEO_R1, k, i);
         const T un = (T)a.U[cidx<W, NV>(nb, v)];
-        const T If = fr.If, omI = T(1) - If;
-        const T uf = isR ? (un * If + u * omI) : (u * If + un * omI);
-        const T au = uf * fr.A;
-        const T x0 = au * fr.n0, x1 = au * fr.n1;
-        if (isR) {
-            g0 = g0 - x0;
-            g1 = g1 - x1;
-        } else {
-            g0 = g0 + x0;
-            g1 = g1 + x1;
-        }
+        const T omI = T(1) - If;
+        const bool isR = (side >> k) & 1;
+        const T uf = u * (isR ? omI : If) + un * (isR ? If : omI);
+        const T au = uf * A;
+        g0 = g0 + au * n0;
+        g1 = g1 + au * n1;
         umin = dmin(umin, un);
         umax = dmax(umax, un);
-        r0[k] = isR ? fr.rR0 : fr.rL0;
-        r1[k] = isR ? fr.rR1 : fr.rL1;
         if (nb >= a.Nint) ghosts |= 1u << k;
-        if constexpr (CFL) lam += (double)fr.A * (fabs(ux * (double)fr.n0 + uy * (double)fr.n1) + c);
+        if constexpr (CFL) lam += (double)A * (fabs(ux * (double)n0 + uy * (double)n1) + c);
     }
     g0 *= iv;
     g1 *= iv;
@@ -295,13 +343,12 @@ __device__ __forceinline__ double pass_a_cell_quad(const PassArgs& a, int i, int
 #pragma unroll
     for (int k = 0; k < MAXDEG; k++) {
         if (k < deg) {
-            FS[((long long)i * MAXDEG + k) * NV + v] = u + phi * (g0 * r0[k] + g1 * r1[k]);
+            const T d = g0 * r0[k] + g1 * r1[k];
+            FS[fs_index<T>(P, k, i, v)] = u + phi * d;
             if ((ghosts >> k) & 1u) {
-                const Slot s = a.slots[(long long)k * a.NcPad + i];
-                const typename PT::Rec fr = PT::rec(a)[s.f & (SLOT_SIDE_R - 1)];
-                const int gc = slot_nb(s);
+                const int gc = a.cnb[k * P + i] & SLOT_NB_MASK;
                 const T ug = (T)a.U[cidx<W, NV>(gc, v)];
-                FS[(long long)gc * (MAXDEG * NV) + v] = ug + phi * ((-g0) * fr.rR0 + (-g1) * fr.rR1);
+                FS[fs_index<T>(P, 0, gc, v)] = ug + phi * d;
             }
         }
     }
@@ -310,17 +357,30 @@ __device__ __forceinline__ double pass_a_cell_quad(const PassArgs& a, int i, int
 }
 
 // LPC lanes per cell (1: one cell per lane; 4: one variable per lane).
+// With one lane per cell the tile's states are staged in LDS first.
 template <typename T, int W, int MAXDEG, bool CFL, bool MULTI, int LPC>
 __global__ __launch_bounds__(256) HCFD_WAVES_A_ATTR void k_pass_a(PassArgs a) {
     __shared__ double sh[256 / WAVE];
-    int t0, step, tend;
-    tile_walk(a, t0, step, tend);
-    for (int tile = t0; tile < tend; tile += step) {
-        const int i = tile * a.tile_cells + (int)threadIdx.x / LPC;
+    __shared__ double2 sU[LPC == 1 ? 2 * 256 : 1];
+    int tfirst, step, tend;
+    tile_walk(a, tfirst, step, tend);
+    for (int tile = tfirst; tile < tend; tile += step) {
+        const int tb = tile * a.tile_cells;
+        const int i = tb + (int)threadIdx.x / LPC;
+        if constexpr (LPC == 1) {
+            __syncthreads();  // the previous tile's readers are done
+            if (i < a.Nc) {
+                double u[NV];
+                load_cell<W, NV>(a.U, i, u);
+                sU[2 * threadIdx.x] = make_double2(u[0], u[1]);
+                sU[2 * threadIdx.x + 1] = make_double2(u[2], u[3]);
+            }
+            __syncthreads();
+        }
         double dti = INFINITY;
         if (i < a.Nint) {
             if constexpr (LPC == 4) dti = pass_a_cell_quad<T, W, MAXDEG, CFL>(a, i, threadIdx.x & 3);
-            else dti = pass_a_cell<T, W, MAXDEG, CFL>(a, i);
+            else dti = pass_a_cell<T, W, MAXDEG, CFL>(a, i, sU, tb, min(a.tile_cells, a.Nc - tb));
         }
         if constexpr (CFL) cfl_tile<MULTI>(a, tile, i, dti, sh);
     }
@@ -371,43 +431,52 @@ __device__ __forceinline__ bool nonphysical(double gma, const double u[NV]) {
 // ---------------------------------------------------------------------------
 // Pass B, FP64
 // ---------------------------------------------------------------------------
+// Face states of the block's tile staged in LDS: FP64 slot k of tile cell l is
+// s[(2k) * 256 + l], s[(2k + 1) * 256 + l]; FP32 slot k is s[k * 256 + l].
+constexpr int LDS_TILE = 256;
+__device__ __forceinline__ void lds_fs(const double2* __restrict__ s, int k, int l, double v[NV]) {
+    const double2 x = s[(2 * k) * LDS_TILE + l], y = s[(2 * k + 1) * LDS_TILE + l];
+    v[0] = x.x; v[1] = x.y; v[2] = y.x; v[3] = y.y;
+}
+__device__ __forceinline__ void lds_fs(const float4* __restrict__ s, int k, int l, float v[NV]) {
+    const float4 x = s[k * LDS_TILE + l];
+    v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w;
+}
+template <typename T, typename S>
+__device__ __forceinline__ void tile_or_global_fs(const T* fs, long long P, const S* __restrict__ s, int tb, int tc,
+                                                  int k, int j, T v[NV]) {
+    const unsigned l = (unsigned)(j - tb);
+    if (l < (unsigned)tc) lds_fs(s, k, (int)l, v);
+    else load_fs(fs, P, k, j, v);
+}
+
 template <int W, int MAXDEG>
-__device__ __forceinline__ void pass_b_cell_fp64(const PassArgs& a, int i, double dt, int inst) {
+__device__ __forceinline__ void pass_b_cell_fp64(const PassArgs& a, int i, double dt, int inst,
+                                                 const double2* __restrict__ sF, int tb, int tc) {
+    const long long P = a.NcPad;
     double u[NV];
     load_cell<W, NV>(a.U, i, u);
     const double iv = a.iv[i];
     double R[NV] = {0.0, 0.0, 0.0, 0.0};
     int gg[MAXDEG];
-    double gn0[MAXDEG], gn1[MAXDEG];
 #pragma unroll
     for (int k = 0; k < MAXDEG; k++) {
         gg[k] = -1;
-        gn0[k] = 0.0;
-        gn1[k] = 0.0;
-        const Slot s = a.slots[(long long)k * a.NcPad + i];
-        if (s.f < 0) continue;
-        const bool isR = (s.f & SLOT_SIDE_R) != 0;
-        const int f = s.f & (SLOT_SIDE_R - 1);
-        const double2* qo = reinterpret_cast<const double2*>(a.FS + ((long long)i * MAXDEG + k) * NV);
-        const double2* qn =
-            reinterpret_cast<const double2*>(a.FS + ((long long)slot_nb(s) * MAXDEG + slot_twin(s)) * NV);
-        const double2 o0 = qo[0], o1 = qo[1], n0 = qn[0], n1 = qn[1];
-        const double2 l0 = isR ? n0 : o0, l1 = isR ? n1 : o1, r0 = isR ? o0 : n0, r1 = isR ? o1 : n1;
-        const double UL[NV] = {l0.x, l0.y, l1.x, l1.y};
-        const double UR[NV] = {r0.x, r0.y, r1.x, r1.y};
-        const FaceN64 fn = a.fn[f];
+        const int nbx = a.cnb[k * P + i];
+        if (nbx == SLOT_EMPTY) continue;
+        const int nb = nbx & SLOT_NB_MASK;
+        const int tw = (int)((unsigned)nbx >> SLOT_TWIN_SHIFT);
+        double Uo[NV], Un[NV];
+        lds_fs(sF, k, i - tb, Uo);
+        tile_or_global_fs(a.FS, P, sF, tb, tc, tw, nb, Un);
+        const double A = geo<MAXDEG>(a.cgeo, P, GEO_A, k, i);
+        const double n0 = geo<MAXDEG>(a.cgeo, P, GEO_N0, k, i);
+        const double n1 = geo<MAXDEG>(a.cgeo, P, GEO_N1, k, i);
         double F[NV];
-        rusanov(a.gamma, UL, UR, fn.n0, fn.n1, F);
+        rusanov(a.gamma, Uo, Un, n0, n1, F);  // outward flux of this cell
 #pragma unroll
-        for (int v = 0; v < NV; v++) {
-            if (isR) R[v] = R[v] + fn.A * F[v];
-            else R[v] = R[v] - fn.A * F[v];
-        }
-        if (fn.ghost >= 0) {
-            gg[k] = fn.ghost;
-            gn0[k] = fn.n0;
-            gn1[k] = fn.n1;
-        }
+        for (int v = 0; v < NV; v++) R[v] = R[v] - A * F[v];
+        if (nb >= a.Nint) gg[k] = nb;
     }
     double unew[NV];
 #pragma unroll
@@ -420,9 +489,11 @@ __device__ __forceinline__ void pass_b_cell_fp64(const PassArgs& a, int i, doubl
 #pragma unroll
     for (int k = 0; k < MAXDEG; k++) {
         if (gg[k] >= 0) {
+            // boundary faces have the interior cell on the L side: n_out = n
             const BCEntry e = a.bctab[a.ghost_bc[gg[k] - a.Nint]];
             double ug[NV];
-            ghost_state(e, a.gamma, unew, gn0[k], gn1[k], ug);
+            ghost_state(e, a.gamma, unew, geo<MAXDEG>(a.cgeo, P, GEO_N0, k, i), geo<MAXDEG>(a.cgeo, P, GEO_N1, k, i),
+                        ug);
             store_cell<W, NV>(a.Unext, gg[k], ug);
         }
     }
@@ -444,36 +515,31 @@ struct Kahan {
 };
 
 template <int W, int MAXDEG>
-__device__ __forceinline__ void pass_b_cell_mixed(const PassArgs& a, int i, double dt, int inst) {
+__device__ __forceinline__ void pass_b_cell_mixed(const PassArgs& a, int i, double dt, int inst,
+                                                  const float4* __restrict__ sF, int tb, int tc) {
+    const long long P = a.NcPad;
     double u[NV];
     load_cell<W, NV>(a.U, i, u);
     const float iv = a.iv32[i];
     const float gma = (float)a.gamma;
     Kahan R[NV];
-    int gg[MAXDEG], gf[MAXDEG];
+    int gg[MAXDEG];
 #pragma unroll
     for (int k = 0; k < MAXDEG; k++) {
         gg[k] = -1;
-        gf[k] = 0;
-        const Slot s = a.slots[(long long)k * a.NcPad + i];
-        if (s.f < 0) continue;
-        const bool isR = (s.f & SLOT_SIDE_R) != 0;
-        const int f = s.f & (SLOT_SIDE_R - 1);
-        const float4 qo = *reinterpret_cast<const float4*>(a.FS32 + ((long long)i * MAXDEG + k) * NV);
-        const float4 qn =
-            *reinterpret_cast<const float4*>(a.FS32 + ((long long)slot_nb(s) * MAXDEG + slot_twin(s)) * NV);
-        const float4 qL = isR ? qn : qo, qR = isR ? qo : qn;
-        const float UL[NV] = {qL.x, qL.y, qL.z, qL.w};
-        const float UR[NV] = {qR.x, qR.y, qR.z, qR.w};
-        const FaceN32 fn = a.fn32[f];
+        const int nbx = a.cnb[k * P + i];
+        if (nbx == SLOT_EMPTY) continue;
+        const int nb = nbx & SLOT_NB_MASK;
+        const int tw = (int)((unsigned)nbx >> SLOT_TWIN_SHIFT);
+        float Uo[NV], Un[NV];
+        lds_fs(sF, k, i - tb, Uo);
+        tile_or_global_fs(a.FS32, P, sF, tb, tc, tw, nb, Un);
+        const float A = geo<MAXDEG>(a.cgeo32, P, GEO_A, k, i);
         float F[NV];
-        rusanov(gma, UL, UR, fn.n0, fn.n1, F);
+        rusanov(gma, Uo, Un, geo<MAXDEG>(a.cgeo32, P, GEO_N0, k, i), geo<MAXDEG>(a.cgeo32, P, GEO_N1, k, i), F);
 #pragma unroll
-        for (int v = 0; v < NV; v++) R[v].add(isR ? fn.A * F[v] : -(fn.A * F[v]));
-        if (fn.ghost >= 0) {
-            gg[k] = fn.ghost;
-            gf[k] = f;
-        }
+        for (int v = 0; v < NV; v++) R[v].add(-(A * F[v]));
+        if (nb >= a.Nint) gg[k] = nb;
     }
     double unew[NV];
 #pragma unroll
@@ -483,25 +549,41 @@ __device__ __forceinline__ void pass_b_cell_mixed(const PassArgs& a, int i, doub
 #pragma unroll
     for (int k = 0; k < MAXDEG; k++) {
         if (gg[k] >= 0) {
-            const FaceN64 f64 = a.fn[gf[k]];  // FP64 normal for the ghost state
+            // FP64 normal for the ghost state (interior cell on the L side)
             const BCEntry e = a.bctab[a.ghost_bc[gg[k] - a.Nint]];
             double ug[NV];
-            ghost_state(e, a.gamma, unew, f64.n0, f64.n1, ug);
+            ghost_state(e, a.gamma, unew, geo<MAXDEG>(a.cgeo, P, GEO_N0, k, i), geo<MAXDEG>(a.cgeo, P, GEO_N1, k, i),
+                        ug);
             store_cell<W, NV>(a.Unext, gg[k], ug);
         }
     }
 }
 
+// The tile's face states (every slot of every cell of the tile, ghosts
+// included) are staged in LDS; neighbours inside the tile are read from there.
 template <bool MIXED, int W, int MAXDEG, bool MULTI>
 __global__ __launch_bounds__(256) HCFD_WAVES_B_ATTR void k_pass_b(PassArgs a) {
-    int t0, step, tend;
-    tile_walk(a, t0, step, tend);
-    for (int tile = t0; tile < tend; tile += step) {
-        const int i = tile * a.tile_cells + threadIdx.x;
+    using S = typename std::conditional<MIXED, float4, double2>::type;
+    constexpr int PLANES = MIXED ? MAXDEG : 2 * MAXDEG;
+    __shared__ S sF[PLANES * LDS_TILE];
+    const long long P = a.NcPad;
+    int tfirst, step, tend;
+    tile_walk(a, tfirst, step, tend);
+    for (int tile = tfirst; tile < tend; tile += step) {
+        const int tb = tile * a.tile_cells;
+        const int i = tb + threadIdx.x;
+        __syncthreads();  // the previous tile's readers are done
+        if (i < a.Nc) {
+            const S* src = reinterpret_cast<const S*>(MIXED ? (const void*)a.FS32 : (const void*)a.FS);
+#pragma unroll
+            for (int q = 0; q < PLANES; q++) sF[q * LDS_TILE + threadIdx.x] = src[q * P + i];
+        }
+        __syncthreads();
         if (i >= a.Nint) continue;
+        const int tc = min(a.tile_cells, a.Nc - tb);
         const int inst = MULTI ? instance_of(a.inst_off, a.ninst, i) : 0;
-        if constexpr (MIXED) pass_b_cell_mixed<W, MAXDEG>(a, i, a.dt[inst], inst);
-        else pass_b_cell_fp64<W, MAXDEG>(a, i, a.dt[inst], inst);
+        if constexpr (MIXED) pass_b_cell_mixed<W, MAXDEG>(a, i, a.dt[inst], inst, sF, tb, tc);
+        else pass_b_cell_fp64<W, MAXDEG>(a, i, a.dt[inst], inst, sF, tb, tc);
     }
 }
 

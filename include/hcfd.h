@@ -177,7 +177,7 @@ int hcfd_stage_update(hcfd_ctx* ctx, double dt);
  * src/Base/Euler.cpp:138-142). */
 int hcfd_stage_bc_cells(hcfd_ctx* ctx);
 /* One BoundaryCondition object's update_cells / update_grads / update_limiter
- * (src/Base/BoundaryConditions/*.cpp) on its own faces: BC ordinal bc (-1 = all).
+ * (src/Base/BoundaryConditions/{Symmetry,Wall,Inlet,Outlet}.cpp) on its own faces: BC ordinal bc (-1 = all).
  * what: 0 = cells, 1 = gradients, 2 = limiter. */
 int hcfd_stage_bc(hcfd_ctx* ctx, int what, int bc);
 /* Ratio of specific heats used by the stage flux (Flux::Flux(nvars, gma, ...),
