@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Benchmark of the explicit unstructured finite-volume Euler residual on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c3|c3mixed|c4|c5]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c3|c3mixed|c4|c5] [--check N]
 
 A step is one residual evaluation plus the forward-Euler update (src/Base/Euler.cpp:85-97):
 pass A (gradient + limiter + CFL bound), the CFL reduction, pass B (flux + update +
@@ -222,24 +222,135 @@ def secondary_c3(steps=None):
     return out
 
 
+def run_batch(name: str, steps: int, warmup: int, tune: bool, check: int, compare_fp64: bool):
+    """Batched wedges (one launch per pass for all instances, hipGraph replay of the
+    steps, autotuned schedule).  value = interior cells of all instances x steps / wall."""
+    import numpy as np
+    from cfd_optimisations_amd import cases as C
+    from cfd_optimisations_amd import lib as L
+    key = {"c4": "c4_batch_fp64", "c5": "c5_batch_mixed"}[name]
+    w = C.WORKLOADS[key]
+    steps = steps or w["steps"]
+    cfl = w["cfl"]
+    prec = w["precision"]
+    t0 = time.perf_counter()
+    specs = C.wedge_batch_specs(w["n"], w["seed"], w["mach"], w["angle"], w["tris"])
+    cases = [C.ramp_case(sp["nx"], sp["ny"], sp["mach"], sp["angle"], seed=sp["seed"]) for sp in specs]
+    gen_s = time.perf_counter() - t0
+    flags = L.FLAG_GRAPH | L.FLAG_ORDER_SFC
+
+    def one(precision):
+        t1 = time.perf_counter()
+        ctx = C.make_batch(cases, precision, flags)
+        setup = time.perf_counter() - t1
+        inf = ctx.info()
+        order = np.argsort(ctx.inst_sizes)
+        U0 = {int(k): ctx.inst_state(int(k)) for k in order[:check]}
+        sched = ctx.autotune(steps=4, cfl=cfl) if tune else ctx.schedule()
+        ctx.step(warmup, cfl=cfl)
+        t1 = time.perf_counter()
+        ctx.step(steps, cfl=cfl)          # graph replays; returns after the stream drained
+        wall = time.perf_counter() - t1
+        nev = min(steps, 20)
+        tm = ctx.time_steps(nev, cfl=cfl)   # per-kernel event timing, eager launches
+        tb, dtb, fb = ctx.status()
+        rec = {"value": inf.Nint * steps / wall, "ms_per_step": 1e3 * wall / steps, "Nc": inf.Nc,
+               "Nint": inf.Nint, "Nf": inf.Nf, "setup_s": round(setup, 2), "schedule": sched,
+               "kernel_ms": {"pass_a": tm["pass_a_ms"], "pass_b": tm["pass_b_ms"], "dt_reduce": tm["dt_reduce_ms"]},
+               "faulted_instances": int(fb.sum()),
+               "faulted": [{"instance": int(k), "mach": round(specs[k]["mach"], 3),
+                            "angle": round(specs[k]["angle"], 2), "triangles": 2 * specs[k]["nx"] * specs[k]["ny"]}
+                           for k in np.nonzero(fb)[0]],
+               "t_range": [float(tb[fb == 0].min()), float(tb[fb == 0].max())] if (fb == 0).any() else None,
+               "device_bytes": inf.device_bytes, "steps_total": warmup + steps + nev}
+        return ctx, rec, U0
+
+    ctx, rec, U0 = one(prec)
+    a_b, b_b, step_b = alg_bytes(rec["Nc"], rec["Nf"], prec == L.MIXED)
+    kern = {"pass_a": (rec["kernel_ms"]["pass_a"], a_b), "pass_b": (rec["kernel_ms"]["pass_b"], b_b)}
+    dom = max(kern, key=lambda k: kern[k][0])
+    achieved = kern[dom][1] / (kern[dom][0] * 1e-3) / 1e9
+    line = {
+        "metric": METRIC, "value": rec["value"], "unit": "cell-updates/s", "n_gpus": 1, "steps": steps,
+        "warmup": warmup, "ms_per_step": rec["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64" if prec == L.MIXED else "f64", "data": "synthetic",
+        "config": {"workload": f"{name}: {w['n']} wedges, Mach U{list(w['mach'])}, angle U{list(w['angle'])} deg, "
+                               f"log-U{[int(x) for x in w['tris']]} triangles (seed {w['seed']}), one launch per pass, "
+                               f"hipGraph replay, {'autotuned' if tune else 'default'} schedule, CFL {cfl}",
+                   "instances": w["n"], "generate_s": round(gen_s, 1), **{k: rec[k] for k in
+                   ("Nc", "Nint", "Nf", "setup_s", "schedule", "device_bytes")}},
+        "hbm_pct_step": 100.0 * step_b / (rec["ms_per_step"] * 1e-3) / (HBM_PEAK_GBS * 1e9),
+        "kernel_ms": rec["kernel_ms"],
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "alg_bytes": kern[dom][1], "traffic": None,
+                     "timing": "event pairs around eager launches right after the graph-timed region"},
+        "faulted_instances": rec["faulted_instances"], "faulted": rec["faulted"], "t_range_ok": rec["t_range"],
+        "cpu_baseline": None,
+    }
+    if check:
+        # the smallest instances against the oracle, from the same initial state, same step count
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as ol
+        agree = []
+        total = rec["steps_total"]
+        for k, u0 in U0.items():
+            c = cases[k]
+            od = ol.OracleDomain(c.hmesh(), c.gamma, c.bctypes, c.qbc)
+            od.zero_bc_iv()
+            Uo, to = od.run(u0, total, cfl=cfl)
+            Ug = ctx.inst_state(k)
+            agree.append({"instance": k, "cells": int(u0.shape[1]), "steps": total,
+                          "bitwise_equal": bool(np.array_equal(Ug, Uo)),
+                          "max_rel": float(np.max(np.abs(Ug - Uo)) / max(np.max(np.abs(Uo)), 1e-300))})
+        line["cpu_agreement"] = agree
+    if compare_fp64 and prec == L.MIXED:
+        Um = [ctx.inst_state(k) for k in range(ctx.ninst)]
+        ctx.close()
+        ctx64, rec64, _ = one(L.FP64)
+        e = []
+        for k in range(ctx64.ninst):
+            Ua = ctx64.inst_state(k)
+            e.append(float(np.linalg.norm(Um[k][0] - Ua[0]) / np.linalg.norm(Ua[0])))
+        ctx64.close()
+        e = np.array(e)
+        line["fp64_batch"] = {"value": rec64["value"], "ms_per_step": rec64["ms_per_step"],
+                              "faulted_instances": rec64["faulted_instances"]}
+        line["speedup_vs_fp64_batch"] = rec["value"] / rec64["value"]
+        fin = np.isfinite(e)
+        line["rho_rel_l2_vs_fp64"] = {"max": float(e[fin].max()) if fin.any() else None,
+                                      "median": float(np.median(e[fin])) if fin.any() else None,
+                                      "non_finite_instances": int((~fin).sum()),
+                                      "per_instance": [float(x) for x in e]}
+    else:
+        ctx.close()
+    return line
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=0, help="timed steps (default: the workload's)")
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c3mixed"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c3mixed", "c4", "c5"])
+    ap.add_argument("--check", type=int, default=0, help="batches: smallest instances checked against the oracle")
+    ap.add_argument("--no-compare", action="store_true", help="c5: skip the FP64 batched comparison run")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample length (0 = skip)")
     ap.add_argument("--graph", action="store_true")
     ap.add_argument("--order", default="sfc", choices=["sfc", "rcm"],
                     help="interior cell order: Hilbert curve of centroids (sfc) or reverse Cuthill-McKee (rcm)")
     ap.add_argument("--tune", action="store_true", help="run the autotuner before timing")
+    ap.add_argument("--no-tune", action="store_true", help="batches: keep the default schedule")
     ap.add_argument("--no-secondary", action="store_true")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus}).",
               file=sys.stderr)
         return 2
-    line = run_single(a.workload, a.steps, a.warmup, a.cpu_seconds, a.graph, not a.no_secondary, a.tune, a.order)
+    if a.workload in ("c4", "c5"):
+        line = run_batch(a.workload, a.steps, a.warmup, not a.no_tune, a.check, not a.no_compare)
+    else:
+        line = run_single(a.workload, a.steps, a.warmup, a.cpu_seconds, a.graph, not a.no_secondary, a.tune,
+                          a.order)
     print(json.dumps(line))
     return 0
 
