@@ -222,6 +222,10 @@ def secondary_c3(steps=None):
     return out
 
 
+def log(msg: str) -> None:
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
 def run_batch(name: str, steps: int, warmup: int, tune: bool, check: int, compare_fp64: bool):
     """Batched wedges (one launch per pass for all instances, hipGraph replay of the
     steps, autotuned schedule).  value = interior cells of all instances x steps / wall."""
@@ -237,6 +241,7 @@ def run_batch(name: str, steps: int, warmup: int, tune: bool, check: int, compar
     specs = C.wedge_batch_specs(w["n"], w["seed"], w["mach"], w["angle"], w["tris"])
     cases = [C.ramp_case(sp["nx"], sp["ny"], sp["mach"], sp["angle"], seed=sp["seed"]) for sp in specs]
     gen_s = time.perf_counter() - t0
+    log(f"{name}: generated {len(cases)} meshes in {gen_s:.1f} s")
     flags = L.FLAG_GRAPH | L.FLAG_ORDER_SFC
 
     def one(precision):
@@ -244,13 +249,16 @@ def run_batch(name: str, steps: int, warmup: int, tune: bool, check: int, compar
         ctx = C.make_batch(cases, precision, flags)
         setup = time.perf_counter() - t1
         inf = ctx.info()
+        log(f"{'mixed' if precision else 'fp64'} batch: {inf.Nint:,} interior cells, built in {setup:.1f} s")
         order = np.argsort(ctx.inst_sizes)
         U0 = {int(k): ctx.inst_state(int(k)) for k in order[:check]}
         sched = ctx.autotune(steps=4, cfl=cfl) if tune else ctx.schedule()
+        log(f"schedule {sched}")
         ctx.step(warmup, cfl=cfl)
         t1 = time.perf_counter()
         ctx.step(steps, cfl=cfl)          # graph replays; returns after the stream drained
         wall = time.perf_counter() - t1
+        log(f"{steps} steps in {wall:.2f} s")
         nev = min(steps, 20)
         tm = ctx.time_steps(nev, cfl=cfl)   # per-kernel event timing, eager launches
         tb, dtb, fb = ctx.status()
@@ -294,6 +302,7 @@ def run_batch(name: str, steps: int, warmup: int, tune: bool, check: int, compar
         agree = []
         total = rec["steps_total"]
         for k, u0 in U0.items():
+            log(f"oracle check of instance {k}")
             c = cases[k]
             od = ol.OracleDomain(c.hmesh(), c.gamma, c.bctypes, c.qbc)
             od.zero_bc_iv()
