@@ -203,6 +203,38 @@ def test_batch_matches_single_exact():
             assert tb[k] == to and fb[k] == 0
 
 
+def _physical(od, U, ghost):
+    Q = od.prims(U)
+    return bool(np.isfinite(U).all() and (Q[0][~ghost] > 0).all() and (Q[3][~ghost] > 0).all())
+
+
+def test_batch_faulting_instance_is_isolated():
+    """A steep wedge (Mach 3.3, 29 deg) goes non-physical at step 9 under the
+    reference method itself (the oracle does too); in a batch only that instance
+    is flagged, it matches the oracle bit for bit up to the fault, and the other
+    instances stay bit-exact."""
+    calm1 = C.ramp_case(40, 20, 2.0, 10.0, seed=5)
+    steep = C.ramp_case(48, 24, 3.3, 29.0, seed=5)
+    calm2 = C.ramp_case(32, 16, 1.8, 5.0, seed=5)
+    cs = [calm1, steep, calm2]
+    with C.make_batch(cs, L.FP64, 0) as b:
+        b.step(8, cfl=0.5)
+        od, U0 = oracle_for(steep)
+        U8, _ = od.run(U0, 8, cfl=0.5)
+        assert _physical(od, U8, steep.hmesh().ghost_mask())
+        assert np.array_equal(b.inst_state(1), U8)
+        assert not b.status()[2].any()
+        b.step(32, cfl=0.5)
+        _, _, f = b.status()
+        assert list(f) == [0, 1, 0]
+        U40, _ = od.run(U0, 40, cfl=0.5)
+        assert not _physical(od, U40, steep.hmesh().ghost_mask())
+        for k in (0, 2):
+            odk, Uk0 = oracle_for(cs[k])
+            Uk, _ = odk.run(Uk0, 40, cfl=0.5)
+            assert np.array_equal(b.inst_state(k), Uk), k
+
+
 MIXED_RHO_TOL = 1e-5   # north-star bound on the relative L2 density error vs FP64
 
 
