@@ -67,7 +67,7 @@ def build_library(incremental: bool = False, jobs: int = 8, verbose: bool = Fals
         objs.append(o)
         if incremental and os.path.exists(o) and os.path.getmtime(o) > max(os.path.getmtime(s), hdr_time):
             continue
-        cmd = [cc] + HIP_FLAGS + ["-c", s, "-o", o]
+        cmd = [cc] + HIP_FLAGS + os.environ.get("HCFD_EXTRA_HIP_FLAGS", "").split() + ["-c", s, "-o", o]
         if s.endswith(".cpp"):
             cmd = [cc] + HIP_FLAGS + ["-x", "c++", "-c", s, "-o", o]
         jobs_todo.append(cmd)

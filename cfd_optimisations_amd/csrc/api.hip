@@ -19,6 +19,7 @@
 
 #include "common.hpp"
 #include "mesh_build.hpp"
+#include "halo.hpp"
 #include "passes.hpp"
 #include "stages.hpp"
 
@@ -101,7 +102,12 @@ struct hcfd_ctx {
     double* sPhi = nullptr;
     double* sR = nullptr;
     double* backup = nullptr;    // autotuner state backup
-    hcfd_schedule sched{256, 1, 1, 0, 0};
+    hcfd_schedule sched{256, 1, 1, 0, 0, 0};
+    // single-kernel step over halo tiles
+    HaloTables halo;
+    bool fused_ok = false;
+    size_t fused_lds = 0;
+    bool bound_valid = false;  // dt holds the CFL step of the current state (single-kernel path)
     double t_host = 0.0;
     double fixed_dt_loaded = NAN;
     // captured two-step graph
@@ -296,6 +302,16 @@ hcfd_ctx* create_ctx(int n, const hcfd_mesh_view* views, int precision, int flag
     c->bctab = dmalloc<BCEntry>(c->nbctab);
     HIPCHK(hipMemsetAsync(c->bctab, 0, sizeof(BCEntry) * c->nbctab, c->st));
     c->sched.xcd_remap = 1;
+    // halo tiles of the single-kernel step; used when their LDS footprint fits
+    build_halo_tables(c->Nint, c->NcPad, c->deg_bucket, c->m.cnb, 256, c->halo, c->st);
+    c->fused_lds = fused_lds_bytes(precision == HCFD_MIXED, c->deg_bucket, c->halo.smax, c->halo.fsmax,
+                                   c->halo.h1max, c->halo.s2max);
+    {
+        int lim = 0;
+        HIPCHK(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
+        c->fused_ok = c->fused_lds <= (size_t)std::min(lim, 80 * 1024);
+    }
+    c->sched.fused = c->fused_ok ? 1 : 0;  // one kernel per step where the halo tiles fit
     HIPCHK(hipStreamSynchronize(c->st));
     return c.release();
 }
@@ -372,12 +388,13 @@ Launch make_launch(hcfd_ctx* c, bool cfl, int cur, int pass) {
     L.v.width = s.aosoa_width;
     L.v.maxdeg = c->deg_bucket;
     L.v.lpc = pass == 0 ? s.lanes_per_cell : 1;
+    const bool fused = pass == 2;
     PassArgs& a = L.a;
     a.Nint = c->Nint;
     a.Nc = c->Nc;
     a.NcPad = c->NcPad;
     a.ninst = c->ninst;
-    a.tile_cells = s.block / L.v.lpc;
+    a.tile_cells = fused ? 256 : s.block / L.v.lpc;
     a.ntiles = (c->Nint + a.tile_cells - 1) / a.tile_cells;
     a.xcd_remap = s.xcd_remap;
     a.tiles_per_xcd = (a.ntiles + NXCD - 1) / NXCD;
@@ -400,7 +417,18 @@ Launch make_launch(hcfd_ctx* c, bool cfl, int cur, int pass) {
     a.bctab = c->bctab;
     a.ghost_bc = c->m.ghost_bc;
     a.fault = c->fault;
-    L.block = s.block;
+    a.h1_off = c->halo.h1_off;
+    a.h1 = c->halo.h1;
+    a.s_off = c->halo.s_off;
+    a.s_ngt = c->halo.s_ngt;
+    a.s2 = c->halo.s2;
+    a.smax = c->halo.smax;
+    a.fsmax = c->halo.fsmax;
+    a.h1max = c->halo.h1max;
+    a.s2max = c->halo.s2max;
+    a.tacc = c->tdev;
+    a.lastdt = c->lastdt;
+    L.block = fused ? 256 : s.block;
     if (s.persistent > 0) {
         int g = std::min(a.ntiles, s.persistent * c->ncu);
         if (s.xcd_remap) g = std::max(NXCD, g / NXCD * NXCD);
@@ -419,9 +447,33 @@ void load_fixed_dt(hcfd_ctx* c, double dt) {
     c->fixed_dt_loaded = dt;
 }
 
+// Single-kernel path, CFL mode: dt must hold the CFL step of the current state
+// before the first step (afterwards every step reduces the bound of its result).
+void ensure_bound(hcfd_ctx* c, double cfl) {
+    if (!c->sched.fused || !(cfl > 0) || c->bound_valid) return;
+    Launch L = make_launch(c, true, c->cur, 2);
+    HIPCHK(launch_cfl_bound(L.v, L.a, L.grid, c->st));
+    HIPCHK(launch_dt_reduce(L.a, cfl, c->dt, c->tdev, c->lastdt, c->st, false));
+    c->bound_valid = true;
+}
+
 // Enqueues one step reading U[cur]; returns nothing, caller flips cur.
 void enqueue_step(hcfd_ctx* c, int cur, double cfl, hipEvent_t* ev = nullptr) {
     const bool use_cfl = cfl > 0;
+    if (c->sched.fused) {
+        // CFL: the kernel leaves the bound of U^{n+1} per tile; the reduction
+        // gives dt of the next step (the kernel advances the clocks by dt itself)
+        Launch F = make_launch(c, use_cfl, cur, 2);
+        if (ev) HIPCHK(hipEventRecord(ev[0], c->st));
+        HIPCHK(launch_fused_step(F.v, F.a, F.grid, c->fused_lds, c->st));
+        if (ev) HIPCHK(hipEventRecord(ev[1], c->st));
+        if (use_cfl) HIPCHK(launch_dt_reduce(F.a, cfl, c->dt, c->tdev, c->lastdt, c->st, false));
+        if (ev) HIPCHK(hipEventRecord(ev[2], c->st));
+        if (ev) HIPCHK(hipEventRecord(ev[3], c->st));
+        c->bound_valid = use_cfl;
+        return;
+    }
+    c->bound_valid = false;
     Launch L = make_launch(c, use_cfl, cur, 0);
     Launch LB = make_launch(c, use_cfl, cur, 1);
     if (ev) HIPCHK(hipEventRecord(ev[0], c->st));
@@ -464,6 +516,8 @@ void run_steps(hcfd_ctx* c, int nsteps, double dt, double cfl) {
     if (nsteps < 0) fail(HCFD_ERR_ARG, "nsteps < 0");
     if (!(cfl > 0) && !(dt > 0)) fail(HCFD_ERR_ARG, "need dt > 0 or cfl > 0");
     if (!(cfl > 0)) load_fixed_dt(c, dt);
+    else c->fixed_dt_loaded = NAN;  // the CFL reduction overwrites dt
+    ensure_bound(c, cfl);
     int done = 0;
     if ((c->flags & HCFD_FLAG_GRAPH) && nsteps >= 2) {
         ensure_graph(c, dt, cfl);
@@ -538,6 +592,7 @@ void change_width(hcfd_ctx* c, int neww) {
     HIPCHK(hipStreamSynchronize(c->st));
     (void)hipFree(d_ident);
     c->sched.aosoa_width = neww;
+    c->bound_valid = false;
     drop_graph(c);
 }
 
@@ -547,7 +602,10 @@ void check_schedule(const hcfd_ctx* c, const hcfd_schedule& s) {
     if (s.aosoa_width != 1 && s.aosoa_width != 8 && s.aosoa_width != 64)
         fail(HCFD_ERR_ARG, "aosoa_width must be 1, 8 or 64");
     if (s.persistent < 0 || s.persistent > 64) fail(HCFD_ERR_ARG, "persistent must be in [0, 64]");
-    (void)c;
+    if (s.fused != 0 && s.fused != 1) fail(HCFD_ERR_ARG, "fused must be 0 or 1");
+    if (s.fused && (s.block != 256 || s.lanes_per_cell != 1))
+        fail(HCFD_ERR_ARG, "the single-kernel step needs block 256 and one lane per cell");
+    if (s.fused && !c->fused_ok) fail(HCFD_ERR_ARG, "this mesh's halo tiles do not fit in LDS (fused unavailable)");
 }
 
 }  // namespace
@@ -556,6 +614,7 @@ hcfd_ctx::~hcfd_ctx() {
     if (st) (void)hipStreamSynchronize(st);
     if (gexec) (void)hipGraphExecDestroy(gexec);
     free_device_mesh(m);
+    free_halo_tables(halo);
     for (void* p : {(void*)bctab, (void*)d_old_of_new, (void*)d_ghost_inst, (void*)U[0], (void*)U[1], (void*)FS,
                     (void*)FS32, (void*)cgeo32, (void*)iv32, (void*)pmin_first, (void*)pmin_last,
                     (void*)dtacc, (void*)dt, (void*)tdev, (void*)lastdt, (void*)fault, (void*)staging, (void*)sU,
@@ -594,6 +653,7 @@ int hcfd_batch_create(int n, const hcfd_mesh_view* meshes, const hcfd_case* case
         if (!(gamma > 1.0)) fail(HCFD_ERR_ARG, "gamma must exceed 1");
         std::unique_ptr<hcfd_ctx> c(create_ctx(n, meshes, precision, flags));
         c->gamma = gamma;
+        c->bound_valid = false;
         std::vector<BCEntry> tab(c->nbctab);
         std::vector<double> uic(4 * (size_t)n);
         for (int k = 0; k < n; k++) {
@@ -646,6 +706,9 @@ int hcfd_get_info(hcfd_ctx* c, hcfd_info* o) {
         b += 8LL * c->Nc * 8 + 16LL * c->Nf + 16LL * c->Nc;  // staging, xf, xc
         o->device_bytes = b;
         o->sched = c->sched;
+        o->fused_available = c->fused_ok ? 1 : 0;
+        o->fused_lds_bytes = (int)c->fused_lds;
+        o->halo_max_cells = c->halo.smax;
     });
 }
 
@@ -724,6 +787,7 @@ int hcfd_initialise(hcfd_ctx* c, const double qic[4]) {
         HIPCHK(launch_init_uniform(c->sched.aosoa_width, c->Nc, c->staging, c->U[c->cur], c->st));
         apply_bc_cells(c, c->U[c->cur]);
         reset_time(c);
+        c->bound_valid = false;
         HIPCHK(hipStreamSynchronize(c->st));
     });
 }
@@ -737,6 +801,7 @@ int hcfd_initialise_prims(hcfd_ctx* c, const double* const Q[4]) {
                                  c->U[c->cur], c->st));
         apply_bc_cells(c, c->U[c->cur]);
         reset_time(c);
+        c->bound_valid = false;
         HIPCHK(hipStreamSynchronize(c->st));
     });
 }
@@ -746,6 +811,7 @@ int hcfd_set_state(hcfd_ctx* c, const double* const U[4]) {
         if (!c || !U) fail(HCFD_ERR_ARG, "null");
         upload_soa(c, U, 1);
         HIPCHK(launch_import(c->sched.aosoa_width, NV, c->Nc, c->d_old_of_new, c->staging, c->Nc, c->U[c->cur], c->st));
+        c->bound_valid = false;
         HIPCHK(hipStreamSynchronize(c->st));
     });
 }
@@ -824,6 +890,7 @@ int hcfd_set_gamma(hcfd_ctx* c, double gamma) {
         if (!(gamma > 1.0)) fail(HCFD_ERR_ARG, "gamma must exceed 1");
         if (c->bcs_set && gamma != c->gamma) fail(HCFD_ERR_STATE, "gamma differs from the one the BCs were set with");
         c->gamma = gamma;
+        c->bound_valid = false;
         drop_graph(c);
     });
 }
@@ -862,6 +929,8 @@ int hcfd_time_steps(hcfd_ctx* c, int nsteps, double dt, double cfl, float* ms_ou
         if (!c || !ms_out || nsteps <= 0) fail(HCFD_ERR_ARG, "bad argument");
         require_bcs(c);
         if (!(cfl > 0)) load_fixed_dt(c, dt);
+        else c->fixed_dt_loaded = NAN;
+        ensure_bound(c, cfl);
         // One timed region: nsteps eager steps with an event pair around every
         // launch; wall = first to last event.
         std::vector<hipEvent_t> ev(4 * nsteps);
@@ -938,7 +1007,7 @@ int hcfd_copy_bandwidth(long long bytes, int reps, double* gbs) {
 int hcfd_set_schedule(hcfd_ctx* c, const hcfd_schedule* s) {
     return guarded([&] {
         if (!c) fail(HCFD_ERR_ARG, "null");
-        hcfd_schedule n = s ? *s : hcfd_schedule{256, 1, 1, 0, 1};
+        hcfd_schedule n = s ? *s : hcfd_schedule{256, 1, 1, 0, 1, c->fused_ok ? 1 : 0};
         check_schedule(c, n);
         change_width(c, n.aosoa_width);
         c->sched = n;
@@ -969,8 +1038,8 @@ int hcfd_autotune(hcfd_ctx* c, int steps, double cfl, const char* cache_path, hc
                 auto bar = line.find('|');
                 if (bar == std::string::npos || line.substr(0, bar) != key) continue;
                 hcfd_schedule s{};
-                if (sscanf(line.c_str() + bar + 1, "%d %d %d %d %d", &s.block, &s.lanes_per_cell, &s.aosoa_width,
-                           &s.persistent, &s.xcd_remap) == 5) {
+                if (sscanf(line.c_str() + bar + 1, "%d %d %d %d %d %d", &s.block, &s.lanes_per_cell,
+                           &s.aosoa_width, &s.persistent, &s.xcd_remap, &s.fused) == 6) {
                     check_schedule(c, s);
                     change_width(c, s.aosoa_width);
                     c->sched = s;
@@ -982,13 +1051,23 @@ int hcfd_autotune(hcfd_ctx* c, int steps, double cfl, const char* cache_path, hc
         }
         const size_t P = c->NcPad;
         if (!c->backup) c->backup = dmalloc<double>(4 * P);
+        // clocks, last dt and fault flags are restored after the probes
+        std::vector<double> t0(c->ninst), l0(c->ninst);
+        std::vector<int> f0(c->ninst);
+        HIPCHK(hipMemcpyAsync(t0.data(), c->tdev, sizeof(double) * c->ninst, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipMemcpyAsync(l0.data(), c->lastdt, sizeof(double) * c->ninst, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipMemcpyAsync(f0.data(), c->fault, sizeof(int) * c->ninst, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
         const double dt_probe = 1e-12;  // only used when cfl <= 0: a negligible step keeps the state valid
         std::vector<hcfd_schedule> cands;
         for (int w : {1, 8, 64})
             for (int b : {64, 128, 256})
                 for (int lpc : {1, 4})
                     for (int pers : {0, 8})
-                        for (int x : {0, 1}) cands.push_back(hcfd_schedule{b, lpc, w, pers, x});
+                        for (int x : {0, 1}) {
+                            cands.push_back(hcfd_schedule{b, lpc, w, pers, x, 0});
+                            if (c->fused_ok && b == 256 && lpc == 1) cands.push_back(hcfd_schedule{b, lpc, w, pers, x, 1});
+                        }
         const double t_save = c->t_host;
         hipEvent_t e0, e1;
         HIPCHK(hipEventCreate(&e0));
@@ -998,6 +1077,7 @@ int hcfd_autotune(hcfd_ctx* c, int steps, double cfl, const char* cache_path, hc
         for (const auto& s : cands) {
             change_width(c, s.aosoa_width);
             c->sched = s;
+            c->bound_valid = false;
             drop_graph(c);
             HIPCHK(hipMemcpyAsync(c->backup, c->U[c->cur], sizeof(double) * 4 * P, hipMemcpyDeviceToDevice, c->st));
             const int cur0 = c->cur;
@@ -1023,18 +1103,16 @@ int hcfd_autotune(hcfd_ctx* c, int steps, double cfl, const char* cache_path, hc
         drop_graph(c);
         c->t_host = t_save;
         // dt/time bookkeeping of the probes is discarded
-        HIPCHK(hipMemsetAsync(c->tdev, 0, sizeof(double) * c->ninst, c->st));
-        HIPCHK(hipMemsetAsync(c->fault, 0, sizeof(int) * c->ninst, c->st));
+        HIPCHK(hipMemcpyAsync(c->tdev, t0.data(), sizeof(double) * c->ninst, hipMemcpyHostToDevice, c->st));
+        HIPCHK(hipMemcpyAsync(c->lastdt, l0.data(), sizeof(double) * c->ninst, hipMemcpyHostToDevice, c->st));
+        HIPCHK(hipMemcpyAsync(c->fault, f0.data(), sizeof(int) * c->ninst, hipMemcpyHostToDevice, c->st));
         c->fixed_dt_loaded = NAN;
+        c->bound_valid = false;
         HIPCHK(hipStreamSynchronize(c->st));
-        if (t_save != 0.0) {
-            std::vector<double> tt(c->ninst, t_save);
-            HIPCHK(hipMemcpy(c->tdev, tt.data(), sizeof(double) * c->ninst, hipMemcpyHostToDevice));
-        }
         if (cache_path) {
             std::ofstream out(cache_path, std::ios::app);
             out << key << "|" << bs.block << " " << bs.lanes_per_cell << " " << bs.aosoa_width << " "
-                << bs.persistent << " " << bs.xcd_remap << "  # " << best_ms / steps << " ms/step\n";
+                << bs.persistent << " " << bs.xcd_remap << " " << bs.fused << "  # " << best_ms / steps << " ms/step\n";
         }
         if (best) *best = bs;
     });
@@ -1074,6 +1152,7 @@ int hcfd_batch_set_state(hcfd_ctx* c, int inst, const double* const U[4]) {
             HIPCHK(hipMemcpyAsync(c->staging + (size_t)v * c->Nc + o, U[v], sizeof(double) * n, hipMemcpyHostToDevice,
                                   c->st));
         HIPCHK(launch_import(c->sched.aosoa_width, NV, c->Nc, c->d_old_of_new, c->staging, c->Nc, c->U[c->cur], c->st));
+        c->bound_valid = false;
         HIPCHK(hipStreamSynchronize(c->st));
     });
 }

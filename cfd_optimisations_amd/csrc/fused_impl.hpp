@@ -1,0 +1,461 @@
+// fused_impl.hpp -- the single-kernel step over halo tiles (included by
+// fused_fp64.hip and fused_mixed.hip).
+//
+// One block owns one tile T of 256 consecutive interior cells and performs a
+// whole step of Euler::run (src/Base/Euler.cpp:85-97) for it:
+//   1. stage in LDS the states U^n of T, of its halo ring H1 and of their
+//      other neighbours S2 (halo.hpp);
+//   2. for every cell of T and H1: Green-Gauss gradient (GaussGreen.cpp:14-40),
+//      min/max limiter (Limiter.cpp:19-77) and the MUSCL face state of each of
+//      its faces (Flux.cpp:83-87), into LDS; cells of T also produce the ghost
+//      side of their boundary faces (BoundaryConditions/Base.cpp:29-53);
+//   3. for every cell of T: Rusanov flux of each face from the two staged face
+//      states (Flux.cpp:52-109), R * 1/V, U^{n+1} = U^n + dt R
+//      (Euler.cpp:107-113), the ghost states of its boundary faces
+//      (BoundaryConditions/{Symmetry,Wall,Inlet,Outlet}.cpp) and, in CFL mode,
+//      the CFL bound of U^{n+1}: the tile partials give dt of the next step.
+// H1 cells are computed by every tile that borders them, with the same
+// arithmetic as in their own tile, so each face state equals the one the
+// two-pass kernels produce (passes_impl.hpp) bit for bit; the results are those
+// of the reference.  U^n is read and U^{n+1} written once per step; face states
+// never leave the CU.
+#pragma once
+#include <type_traits>
+
+#include "halo.hpp"
+#include "passes_impl.hpp"
+
+namespace hcfd {
+
+constexpr int FUSED_TILE = 256;
+
+// Local index of cell j among the tile's staged cells, or -1.
+struct TileLists {
+    int tb, tc, nh1, ngt, ns2, Nint;
+    const int* sH1;
+    const int* sS2;
+};
+
+__device__ __forceinline__ int lds_lower_bound(const int* a, int n, int x) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] < x) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ int local_of(const TileLists& L, int j) {
+    const unsigned d = (unsigned)(j - L.tb);
+    if (d < (unsigned)L.tc) return (int)d;
+    if (j < L.Nint) {
+        const int p = lds_lower_bound(L.sH1, L.nh1, j);
+        if (p < L.nh1 && L.sH1[p] == j) return L.tc + p;
+    } else {
+        const int p = lds_lower_bound(L.sS2, L.ngt, j);
+        if (p < L.ngt && L.sS2[p] == j) return L.tc + L.nh1 + p;
+    }
+    const int nr = L.ns2 - L.ngt;
+    const int p = lds_lower_bound(L.sS2 + L.ngt, nr, j);
+    if (p < nr && L.sS2[L.ngt + p] == j) return L.tc + L.nh1 + L.ngt + p;
+    return -1;
+}
+
+// LDS element types: FP64 cell state / face state = two double2; FP32 = one float4.
+template <typename T>
+struct FusedTypes;
+template <>
+struct FusedTypes<double> {
+    using V = double2;
+    static constexpr int PER = 2;  // 16-byte units per state
+    __device__ static void put(V* s, int pitch, int l, const double u[NV]) {
+        s[l] = make_double2(u[0], u[1]);
+        s[pitch + l] = make_double2(u[2], u[3]);
+    }
+    __device__ static void get(const V* s, int pitch, int l, double u[NV]) {
+        const double2 x = s[l], y = s[pitch + l];
+        u[0] = x.x; u[1] = x.y; u[2] = y.x; u[3] = y.y;
+    }
+};
+template <>
+struct FusedTypes<float> {
+    using V = float4;
+    static constexpr int PER = 1;
+    __device__ static void put(V* s, int, int l, const float u[NV]) { s[l] = make_float4(u[0], u[1], u[2], u[3]); }
+    __device__ static void get(const V* s, int, int l, float u[NV]) {
+        const float4 x = s[l];
+        u[0] = x.x; u[1] = x.y; u[2] = x.z; u[3] = x.w;
+    }
+};
+
+// Per-cell inputs of pass A and pass B, read from global memory once.
+template <typename T, int MAXDEG>
+struct CellIn {
+    int nbx[MAXDEG];  // neighbour | twin << 29, or SLOT_EMPTY
+    T A[MAXDEG], n0[MAXDEG], n1[MAXDEG], If[MAXDEG], r0[MAXDEG], r1[MAXDEG];
+    T iv;
+    int side;
+};
+
+template <typename T, int MAXDEG>
+__device__ __forceinline__ void load_cell_in(const PassArgs& a, int g, CellIn<T, MAXDEG>& c) {
+    const long long P = a.NcPad;
+    const T* __restrict__ gp = PassATypes<T>::geo(a);
+    c.iv = PassATypes<T>::iv(a, g);
+    c.side = a.cside[g];
+#pragma unroll
+    for (int k = 0; k < MAXDEG; k++) {
+        c.nbx[k] = a.cnb[k * P + g];
+        c.A[k] = geo<MAXDEG>(gp, P, GEO_A, k, g);
+        c.n0[k] = geo<MAXDEG>(gp, P, GEO_N0, k, g);
+        c.n1[k] = geo<MAXDEG>(gp, P, GEO_N1, k, g);
+        c.If[k] = geo<MAXDEG>(gp, P, GEO_IF, k, g);
+        c.r0[k] = geo<MAXDEG>(gp, P, GEO_R0, k, g);
+        c.r1[k] = geo<MAXDEG>(gp, P, GEO_R1, k, g);
+    }
+}
+
+// Pass A of one staged cell (local index l): gradient, limiter and the face
+// state of each of its faces into LDS (a tile cell also writes the ghost side
+// of its boundary faces).  lnb_out receives the local neighbour indices.
+// Empty slots carry zero geometry (the packing writes zeros), so skipping them
+// by nbx leaves every sum exactly as in the two-pass kernels.
+template <typename T, int MAXDEG>
+__device__ __forceinline__ void fused_cell_a(const PassArgs& a, const TileLists& L, const CellIn<T, MAXDEG>& c, int l,
+                                             bool own, const typename FusedTypes<T>::V* __restrict__ sU,
+                                             int su_pitch, typename FusedTypes<T>::V* __restrict__ sFS,
+                                             int fs_pitch, int lnb_out[MAXDEG], bool& bad) {
+    using FT = FusedTypes<T>;
+    T u[NV];
+    FT::get(sU, su_pitch, l, u);
+    T g2[NV][2], umin[NV], umax[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+        g2[v][0] = T(0);
+        g2[v][1] = T(0);
+        umin[v] = u[v];
+        umax[v] = u[v];
+    }
+    int ln[MAXDEG];
+    unsigned ghosts = 0;
+    int deg = 0;
+#pragma unroll
+    for (int k = 0; k < MAXDEG; k++) {
+        ln[k] = -1;
+        const int nbx = c.nbx[k];
+        if (nbx == SLOT_EMPTY) continue;
+        deg = k + 1;
+        const int nb = nbx & SLOT_NB_MASK;
+        int q = local_of(L, nb);
+        if (q < 0) {
+            bad = true;
+            q = l;
+        }
+        ln[k] = q;
+        T un[NV];
+        FT::get(sU, su_pitch, q, un);
+        const T If = c.If[k];
+        const T omI = T(1) - If;
+        const bool isR = (c.side >> k) & 1;
+        const T wo = isR ? omI : If, wn = isR ? If : omI;
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            const T uf = u[v] * wo + un[v] * wn;
+            const T au = uf * c.A[k];
+            g2[v][0] = g2[v][0] + au * c.n0[k];
+            g2[v][1] = g2[v][1] + au * c.n1[k];
+            umin[v] = dmin(umin[v], un[v]);
+            umax[v] = dmax(umax[v], un[v]);
+        }
+        if (nb >= a.Nint) ghosts |= 1u << k;
+    }
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+        g2[v][0] *= c.iv;
+        g2[v][1] *= c.iv;
+    }
+    T phi[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+        phi[v] = T(1);
+        const T dmn = umin[v] - u[v], dmx = umax[v] - u[v];
+#pragma unroll
+        for (int k = 0; k < MAXDEG; k++)
+            if (k < deg) phi[v] = dmin(phi[v], limiter_face(g2[v][0] * c.r0[k] + g2[v][1] * c.r1[k], dmn, dmx));
+    }
+#pragma unroll
+    for (int k = 0; k < MAXDEG; k++) {
+        if (k < deg) {
+            T d[NV], uf[NV];
+#pragma unroll
+            for (int v = 0; v < NV; v++) {
+                d[v] = g2[v][0] * c.r0[k] + g2[v][1] * c.r1[k];
+                uf[v] = u[v] + phi[v] * d[v];
+            }
+            FT::put(sFS + (size_t)(k * FT::PER) * fs_pitch, fs_pitch, l, uf);
+            if (own && ((ghosts >> k) & 1u)) {
+                // ghost side of a boundary face: U_g + phi_L ((-G_L) . rR), rR = -rL;
+                // the ghost's slot lies in the face-state region [tc + nh1, fs_pitch)
+                if (ln[k] < L.tc + L.nh1 || ln[k] >= fs_pitch) {
+                    bad = true;
+                    continue;
+                }
+                T ug[NV], ufg[NV];
+                FT::get(sU, su_pitch, ln[k], ug);
+#pragma unroll
+                for (int v = 0; v < NV; v++) ufg[v] = ug[v] + phi[v] * d[v];
+                FT::put(sFS, fs_pitch, ln[k], ufg);
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < MAXDEG; k++) lnb_out[k] = ln[k];
+}
+
+// CFL bound 1 / (iv sum_f A_f (|u.n_f| + c)) of an FP64 state, from a cell's inputs.
+template <typename T, int MAXDEG>
+__device__ __forceinline__ double cfl_bound_in(double gamma, const CellIn<T, MAXDEG>& ci, const double u[NV]) {
+    const double gmo = gamma - 1;
+    const double ir = 1.0 / u[0];
+    const double ux = u[1] * ir, uy = u[2] * ir;
+    const double p = gmo * (u[3] - 0.5 * (u[1] * ux + u[2] * uy));
+    const double c = sqrt(gamma * p * ir);
+    double lam = 0.0;
+#pragma unroll
+    for (int k = 0; k < MAXDEG; k++) {
+        if (ci.nbx[k] == SLOT_EMPTY) continue;
+        lam += (double)ci.A[k] * (fabs(ux * (double)ci.n0[k] + uy * (double)ci.n1[k]) + c);
+    }
+    return 1.0 / ((double)ci.iv * lam);
+}
+
+template <typename T, int MAXDEG>
+__device__ __forceinline__ double cfl_bound(const PassArgs& a, int g, const double u[NV]) {
+    CellIn<T, MAXDEG> ci;
+    load_cell_in<T, MAXDEG>(a, g, ci);
+    return cfl_bound_in<T, MAXDEG>(a.gamma, ci, u);
+}
+
+// CFL partials of one tile without a block barrier: each wavefront folds its
+// minimum into LDS with an atomic, and the last of the tile's wavefronts to
+// arrive (LDS counter) publishes the tile partial and resets the slots.  The
+// bounds are positive (or +inf, or NaN for a faulted cell), so the bit patterns
+// order like the values and NaN never wins, as with fmin.  For batches the
+// tile's first and last instance get their own slot; cells of instances
+// strictly inside the tile use global atomics, as in cfl_tile.
+struct TileMin {
+    unsigned long long first, last;
+    unsigned arrived;
+};
+
+template <bool MULTI>
+__device__ __forceinline__ void cfl_tile_lds(const PassArgs& a, int tile, int i, double dti, TileMin* tm) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int nwaves = FUSED_TILE / WAVE;
+    double vf = dti, vl = INFINITY;
+    if constexpr (MULTI) {
+        const int c0 = tile * FUSED_TILE;
+        const int c1 = min(c0 + FUSED_TILE, a.Nint) - 1;
+        const int bf = instance_of(a.inst_off, a.ninst, c0);
+        const int bl = instance_of(a.inst_off, a.ninst, c1);
+        vf = INFINITY;
+        if (i < a.Nint) {
+            const int bi = instance_of(a.inst_off, a.ninst, i);
+            if (bi == bf) vf = dti;
+            else if (bi == bl) vl = dti;
+            else atomicMin(&a.dtacc[bi], dbits(dti));
+        }
+        vl = wave_min(vl);
+    }
+    vf = wave_min(vf);
+    if (lane == 0) {
+        atomicMin(&tm->first, dbits(vf));
+        if constexpr (MULTI) atomicMin(&tm->last, dbits(vl));
+        __threadfence_block();
+        if (atomicAdd(&tm->arrived, 1u) == nwaves - 1) {
+            __threadfence_block();
+            a.pmin_first[tile] = __longlong_as_double((long long)atomicExch(&tm->first, dbits(INFINITY)));
+            if constexpr (MULTI)
+                a.pmin_last[tile] = __longlong_as_double((long long)atomicExch(&tm->last, dbits(INFINITY)));
+            tm->arrived = 0;
+        }
+    }
+}
+
+// Minimum waves per SIMD for the single-kernel step (bounds its VGPRs; the
+// LDS of a tile allows about three blocks of four waves per CU).
+#ifndef HCFD_WAVES_FUSED
+#define HCFD_WAVES_FUSED 3
+#endif
+
+template <typename T, int W, int MAXDEG, bool CFL, bool MULTI>
+__global__ __launch_bounds__(FUSED_TILE) __attribute__((amdgpu_waves_per_eu(MAXDEG <= 3 ? HCFD_WAVES_FUSED : 2))) void
+k_fused_step(
+    PassArgs a) {
+    using FT = FusedTypes<T>;
+    using V = typename FT::V;
+    extern __shared__ double2 fused_lds[];
+    const int su_pitch = a.smax, fs_pitch = a.fsmax;
+    V* sU = reinterpret_cast<V*>(fused_lds);
+    V* sFS = sU + (size_t)FT::PER * su_pitch;
+    int* sH1 = reinterpret_cast<int*>(sFS + (size_t)FT::PER * MAXDEG * fs_pitch);
+    int* sS2 = sH1 + ((a.h1max + 3) & ~3);
+    TileMin* tmin = reinterpret_cast<TileMin*>(sS2 + ((a.s2max + 3) & ~3));
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        tmin->first = dbits(INFINITY);
+        tmin->last = dbits(INFINITY);
+        tmin->arrived = 0;
+    }  // first read after the tile loop's first barrier
+
+    if constexpr (CFL) {
+        // this step's dt was reduced from the CFL bound of U^n: advance the clocks
+        if (blockIdx.x == 0)
+            for (int b = tid; b < a.ninst; b += blockDim.x) {
+                a.tacc[b] += a.dt[b];
+                a.lastdt[b] = a.dt[b];
+            }
+    }
+    int tfirst, step, tend;
+    tile_walk(a, tfirst, step, tend);
+    for (int tile = tfirst; tile < tend; tile += step) {
+        TileLists L;
+        L.tb = tile * FUSED_TILE;
+        L.tc = min(FUSED_TILE, a.Nint - L.tb);
+        const int h0 = a.h1_off[tile], s0 = a.s_off[tile];
+        L.nh1 = a.h1_off[tile + 1] - h0;
+        L.ns2 = a.s_off[tile + 1] - s0;
+        L.ngt = a.s_ngt[tile];
+        L.Nint = a.Nint;
+        L.sH1 = sH1;
+        L.sS2 = sS2;
+        const int i = L.tb + tid;
+        const bool hasT = tid < L.tc;
+        // this lane's tile cell: its inputs are in flight while the tile is staged
+        CellIn<T, MAXDEG> ci;
+        double uown[NV] = {0, 0, 0, 0};  // U^n of the lane's tile cell (FP64: read back from LDS in pass B)
+        if (hasT) {
+            load_cell_in<T, MAXDEG>(a, i, ci);
+            load_cell<W, NV>(a.U, i, uown);
+        }
+        __syncthreads();  // the previous tile's readers are done
+        if (hasT) {
+            T ut[NV];
+#pragma unroll
+            for (int v = 0; v < NV; v++) ut[v] = (T)uown[v];
+            FT::put(sU, su_pitch, tid, ut);
+        }
+        for (int q = tid; q < L.nh1; q += FUSED_TILE) sH1[q] = a.h1[h0 + q];
+        for (int q = tid; q < L.ns2; q += FUSED_TILE) sS2[q] = a.s2[s0 + q];
+        const int nhalo = L.nh1 + L.ns2;
+        for (int q = tid; q < nhalo; q += FUSED_TILE) {
+            const int g = q < L.nh1 ? a.h1[h0 + q] : a.s2[s0 + q - L.nh1];
+            double ud[NV];
+            load_cell<W, NV>(a.U, g, ud);
+            T ut[NV];
+#pragma unroll
+            for (int v = 0; v < NV; v++) ut[v] = (T)ud[v];
+            FT::put(sU, su_pitch, L.tc + q, ut);
+        }
+        __syncthreads();
+        // pass A: the tile cells, then the halo ring (the last lanes take it)
+        bool bad = false;
+        int lnb[MAXDEG];
+#pragma unroll
+        for (int k = 0; k < MAXDEG; k++) lnb[k] = -1;
+        if (hasT) fused_cell_a<T, MAXDEG>(a, L, ci, tid, true, sU, su_pitch, sFS, fs_pitch, lnb, bad);
+        // the halo ring goes to one wavefront, rotated from tile to tile so the
+        // extra round does not always land on the same SIMD
+        const int rtid = (tid + WAVE * ((blockIdx.x + tile) & (FUSED_TILE / WAVE - 1))) & (FUSED_TILE - 1);
+        for (int q = FUSED_TILE - 1 - rtid; q < L.nh1; q += FUSED_TILE) {
+            CellIn<T, MAXDEG> ch;
+            load_cell_in<T, MAXDEG>(a, a.h1[h0 + q], ch);
+            int ln[MAXDEG];
+            fused_cell_a<T, MAXDEG>(a, L, ch, L.tc + q, false, sU, su_pitch, sFS, fs_pitch, ln, bad);
+        }
+        __syncthreads();
+        // pass B over the tile
+        double dti = INFINITY;
+        if (hasT) {
+            const long long P = a.NcPad;
+            const int inst = MULTI ? instance_of(a.inst_off, a.ninst, i) : 0;
+            const double dt = a.dt[inst];
+            double R[NV] = {0.0, 0.0, 0.0, 0.0};
+            Kahan K[NV];
+            unsigned gmask = 0;
+#pragma unroll
+            for (int k = 0; k < MAXDEG; k++) {
+                const int nbx = ci.nbx[k];
+                if (nbx == SLOT_EMPTY) continue;
+                const int tw = (int)((unsigned)nbx >> SLOT_TWIN_SHIFT);
+                int q = lnb[k];
+                if (q < 0 || q >= fs_pitch || tw >= MAXDEG) {
+                    bad = true;
+                    q = tid;
+                }
+                T Uo[NV], Un[NV];
+                FT::get(sFS + (size_t)(k * FT::PER) * fs_pitch, fs_pitch, tid, Uo);
+                FT::get(sFS + (size_t)((tw < MAXDEG ? tw : 0) * FT::PER) * fs_pitch, fs_pitch, q, Un);
+                T F[NV];
+                rusanov((T)a.gamma, Uo, Un, ci.n0[k], ci.n1[k], F);
+#pragma unroll
+                for (int v = 0; v < NV; v++) {
+                    if constexpr (sizeof(T) == 8) R[v] = R[v] - ci.A[k] * F[v];
+                    else K[v].add(-(ci.A[k] * F[v]));
+                }
+                if ((nbx & SLOT_NB_MASK) >= a.Nint) gmask |= 1u << k;
+            }
+            double unew[NV];
+            if constexpr (sizeof(T) == 8) {
+                FT::get(sU, su_pitch, tid, uown);
+#pragma unroll
+                for (int v = 0; v < NV; v++) {
+                    R[v] = R[v] * (double)ci.iv;
+                    unew[v] = uown[v] + dt * R[v];
+                }
+            } else {
+#pragma unroll
+                for (int v = 0; v < NV; v++) unew[v] = uown[v] + dt * (K[v].value() * (double)ci.iv);
+            }
+            store_cell<W, NV>(a.Unext, i, unew);
+            if (nonphysical(a.gamma, unew) || bad) a.fault[inst] = 1;
+#pragma unroll
+            for (int k = 0; k < MAXDEG; k++) {
+                if ((gmask >> k) & 1u) {
+                    // boundary faces have the interior cell on the L side: n_out = n (FP64)
+                    const int gc = ci.nbx[k] & SLOT_NB_MASK;
+                    const BCEntry e = a.bctab[a.ghost_bc[gc - a.Nint]];
+                    double ug[NV];
+                    ghost_state(e, a.gamma, unew, geo<MAXDEG>(a.cgeo, P, GEO_N0, k, i),
+                                geo<MAXDEG>(a.cgeo, P, GEO_N1, k, i), ug);
+                    store_cell<W, NV>(a.Unext, gc, ug);
+                }
+            }
+            if constexpr (CFL) dti = cfl_bound_in<T, MAXDEG>(a.gamma, ci, unew);
+        } else if (bad) {
+            a.fault[0] = 1;
+        }
+        if constexpr (CFL) cfl_tile_lds<MULTI>(a, tile, i, dti, tmin);
+    }
+}
+
+// CFL bound of the current state U (tile partials only), before the first
+// step of a run: the single-kernel step reduces the bound of U^{n+1} itself.
+template <typename T, int W, int MAXDEG, bool MULTI>
+__global__ __launch_bounds__(FUSED_TILE) void k_cfl_bound(PassArgs a) {
+    __shared__ double sh[FUSED_TILE / WAVE];
+    int tfirst, step, tend;
+    tile_walk(a, tfirst, step, tend);
+    for (int tile = tfirst; tile < tend; tile += step) {
+        const int i = tile * FUSED_TILE + threadIdx.x;
+        double dti = INFINITY;
+        if (i < a.Nint) {
+            double u[NV];
+            load_cell<W, NV>(a.U, i, u);
+            dti = cfl_bound<T, MAXDEG>(a, i, u);
+        }
+        cfl_tile<MULTI>(a, tile, i, dti, sh);
+    }
+}
+
+}  // namespace hcfd

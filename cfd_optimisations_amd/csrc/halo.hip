@@ -1,0 +1,204 @@
+// halo.hip -- device build of the per-tile halo lists (see halo.hpp).
+//
+// Both lists come from one pattern: emit a 64-bit key (tile << 32 | cell) per
+// (cell, slot) pair that qualifies, radix-sort, drop duplicates, and find each
+// tile's range by binary search.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "halo.hpp"
+
+namespace hcfd {
+namespace {
+
+#define HIPCHK(x)                                                                                   \
+    do {                                                                                            \
+        hipError_t e_ = (x);                                                                        \
+        if (e_ != hipSuccess) fail(HCFD_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+using u64 = unsigned long long;
+constexpr u64 NO_KEY = ~0ull;
+constexpr u64 CLS_REST = 1ull << 30;  // S2 entries that are not ghosts of the tile's own cells
+constexpr u64 ID_MASK = (1ull << 30) - 1;
+
+template <typename T>
+T* dalloc(size_t n) {
+    T* p = nullptr;
+    hipError_t e = hipMalloc(&p, sizeof(T) * (n ? n : 1));
+    if (e != hipSuccess) fail(HCFD_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return p;
+}
+
+inline unsigned nblk(long long n) { return (unsigned)((n + 255) / 256); }
+
+__device__ __forceinline__ long long lower_bound_u64(const u64* a, long long n, u64 x) {
+    long long lo = 0, hi = n;
+    while (lo < hi) {
+        const long long mid = (lo + hi) >> 1;
+        if (a[mid] < x) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// H1 candidates: interior neighbours in another tile.
+__global__ void k_h1_keys(int Nint, long long P, int nslot, int TC, const int* __restrict__ cnb, u64* __restrict__ keys) {
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (long long)nslot * Nint) return;
+    const int k = (int)(q / Nint), i = (int)(q % Nint);
+    const int x = cnb[k * P + i];
+    u64 key = NO_KEY;
+    if (x != SLOT_EMPTY) {
+        const int j = x & SLOT_NB_MASK;
+        if (j < Nint && j / TC != i / TC) key = ((u64)(i / TC) << 32) | (u64)j;
+    }
+    keys[q] = key;
+}
+
+// S2 candidates: ghosts of tile cells (class 0), and neighbours of H1 cells
+// that are neither in the tile nor in its H1 list (class 1).
+__global__ void k_s2_keys(int Nint, long long P, int nslot, int TC, const int* __restrict__ cnb,
+                          const u64* __restrict__ h1keys, long long nh1, u64* __restrict__ keys) {
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long n_own = (long long)nslot * Nint;
+    if (q >= n_own + (long long)nslot * nh1) return;
+    u64 key = NO_KEY;
+    if (q < n_own) {
+        const int k = (int)(q / Nint), i = (int)(q % Nint);
+        const int x = cnb[k * P + i];
+        if (x != SLOT_EMPTY) {
+            const int j = x & SLOT_NB_MASK;
+            if (j >= Nint) key = ((u64)(i / TC) << 32) | (u64)j;
+        }
+    } else {
+        const long long r = q - n_own;
+        const int k = (int)(r / nh1);
+        const long long e = r % nh1;
+        const u64 hk = h1keys[e];
+        const u64 t = hk >> 32;
+        const int c = (int)(hk & 0xffffffffull);
+        const int x = cnb[k * P + c];
+        if (x != SLOT_EMPTY) {
+            const int j = x & SLOT_NB_MASK;
+            bool skip = false;
+            if (j < Nint && (u64)(j / TC) == t) skip = true;
+            if (!skip && j < Nint) {
+                const long long p = lower_bound_u64(h1keys, nh1, (t << 32) | (u64)j);
+                skip = p < nh1 && h1keys[p] == ((t << 32) | (u64)j);
+            }
+            if (!skip) key = (t << 32) | CLS_REST | (u64)j;
+        }
+    }
+    keys[q] = key;
+}
+
+__global__ void k_tile_bounds(int ntiles, const u64* __restrict__ keys, long long n, u64 cls, int* __restrict__ off) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > ntiles) return;
+    off[t] = (int)lower_bound_u64(keys, n, ((u64)t << 32) | cls);
+}
+
+__global__ void k_key_ids(long long n, const u64* __restrict__ keys, int* __restrict__ ids) {
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) ids[q] = (int)(keys[q] & ID_MASK);
+}
+
+// Sorts keys[0, n) and removes duplicates and NO_KEY entries; returns the count.
+long long sort_unique(u64*& keys, long long n, hipStream_t st) {
+    if (n > 0x7fffffffLL) fail(HCFD_ERR_ARG, "halo key list too long");
+    u64* alt = dalloc<u64>(n);
+    hipcub::DoubleBuffer<u64> kb(keys, alt);
+    size_t bytes = 0;
+    HIPCHK(hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, kb, (int)n, 0, 64, st));
+    void* tmp = dalloc<char>(bytes);
+    HIPCHK(hipcub::DeviceRadixSort::SortKeys(tmp, bytes, kb, (int)n, 0, 64, st));
+    HIPCHK(hipStreamSynchronize(st));
+    (void)hipFree(tmp);
+    u64* sorted = kb.Current();
+    u64* other = sorted == keys ? alt : keys;
+    int* d_cnt = dalloc<int>(1);
+    bytes = 0;
+    HIPCHK(hipcub::DeviceSelect::Unique(nullptr, bytes, sorted, other, d_cnt, (int)n, st));
+    tmp = dalloc<char>(bytes);
+    HIPCHK(hipcub::DeviceSelect::Unique(tmp, bytes, sorted, other, d_cnt, (int)n, st));
+    int cnt = 0;
+    HIPCHK(hipMemcpyAsync(&cnt, d_cnt, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    (void)hipFree(tmp);
+    (void)hipFree(d_cnt);
+    (void)hipFree(sorted);
+    keys = other;
+    if (cnt > 0) {
+        u64 last = 0;
+        HIPCHK(hipMemcpy(&last, keys + cnt - 1, sizeof(u64), hipMemcpyDeviceToHost));
+        if (last == NO_KEY) cnt--;
+    }
+    return cnt;
+}
+
+}  // namespace
+
+void build_halo_tables(int Nint, int NcPad, int nslot, const int* cnb, int TC, HaloTables& out, hipStream_t st) {
+    free_halo_tables(out);
+    out.TC = TC;
+    out.ntiles = (Nint + TC - 1) / TC;
+    const long long P = NcPad;
+    const long long n1 = (long long)nslot * Nint;
+    // ---- H1 ---------------------------------------------------------------------------
+    u64* k1 = dalloc<u64>(n1);
+    k_h1_keys<<<nblk(n1), 256, 0, st>>>(Nint, P, nslot, TC, cnb, k1);
+    HIPCHK(hipGetLastError());
+    const long long nh1 = sort_unique(k1, n1, st);
+    out.nh1 = nh1;
+    out.h1_off = dalloc<int>(out.ntiles + 1);
+    out.h1 = dalloc<int>(nh1);
+    k_tile_bounds<<<nblk(out.ntiles + 1), 256, 0, st>>>(out.ntiles, k1, nh1, 0, out.h1_off);
+    if (nh1) k_key_ids<<<nblk(nh1), 256, 0, st>>>(nh1, k1, out.h1);
+    // ---- S2 ---------------------------------------------------------------------------
+    const long long n2 = n1 + (long long)nslot * nh1;
+    u64* k2 = dalloc<u64>(n2);
+    k_s2_keys<<<nblk(n2), 256, 0, st>>>(Nint, P, nslot, TC, cnb, k1, nh1, k2);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(st));
+    (void)hipFree(k1);
+    const long long ns2 = sort_unique(k2, n2, st);
+    out.ns2 = ns2;
+    out.s_off = dalloc<int>(out.ntiles + 1);
+    out.s_ngt = dalloc<int>(out.ntiles + 1);
+    out.s2 = dalloc<int>(ns2);
+    k_tile_bounds<<<nblk(out.ntiles + 1), 256, 0, st>>>(out.ntiles, k2, ns2, 0, out.s_off);
+    k_tile_bounds<<<nblk(out.ntiles + 1), 256, 0, st>>>(out.ntiles, k2, ns2, CLS_REST, out.s_ngt);
+    if (ns2) k_key_ids<<<nblk(ns2), 256, 0, st>>>(ns2, k2, out.s2);
+    HIPCHK(hipGetLastError());
+    // ---- sizes (host) ------------------------------------------------------------------
+    std::vector<int> h1o(out.ntiles + 1), so(out.ntiles + 1), sg(out.ntiles + 1);
+    HIPCHK(hipMemcpyAsync(h1o.data(), out.h1_off, sizeof(int) * (out.ntiles + 1), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(so.data(), out.s_off, sizeof(int) * (out.ntiles + 1), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(sg.data(), out.s_ngt, sizeof(int) * (out.ntiles + 1), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    (void)hipFree(k2);
+    for (int t = 0; t < out.ntiles; t++) {
+        sg[t] -= so[t];  // ghosts of the tile's cells: [s_off, first class-1 entry)
+        const int tc = std::min(TC, Nint - t * TC);
+        const int nh = h1o[t + 1] - h1o[t], ns = so[t + 1] - so[t];
+        out.smax = std::max(out.smax, tc + nh + ns);
+        out.fsmax = std::max(out.fsmax, tc + nh + sg[t]);
+        out.h1max = std::max(out.h1max, nh);
+        out.s2max = std::max(out.s2max, ns);
+    }
+    HIPCHK(hipMemcpyAsync(out.s_ngt, sg.data(), sizeof(int) * out.ntiles, hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+}
+
+void free_halo_tables(HaloTables& t) {
+    for (void* p : {(void*)t.h1_off, (void*)t.h1, (void*)t.s_off, (void*)t.s_ngt, (void*)t.s2})
+        if (p) (void)hipFree(p);
+    t = HaloTables{};
+}
+
+}  // namespace hcfd

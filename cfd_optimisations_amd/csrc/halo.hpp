@@ -1,0 +1,49 @@
+// halo.hpp -- per-tile halo lists of the single-kernel step (built on the device).
+//
+// Interior cells are cut into tiles of TC consecutive cells (the block's
+// cells).  For tile t with cells T:
+//   H1(t)  interior cells outside T that share a face with a cell of T; the
+//          block recomputes their gradient, limiter and face states;
+//   S2(t)  the cells whose state is also needed and that are neither in T nor
+//          in H1: first the ghosts of T's boundary faces (they receive a face
+//          state), then the remaining neighbours of H1 cells (interior or ghost).
+// Each list is sorted by cell id.  A block stages the states of T, H1, S2 (in
+// that order: local index = position) and the face states of T, H1 and the
+// ghosts of T in LDS.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+namespace hcfd {
+
+struct HaloTables {
+    int TC = 0, ntiles = 0;
+    int* h1_off = nullptr;  // [ntiles + 1]
+    int* h1 = nullptr;      // H1 cell ids, tile by tile
+    int* s_off = nullptr;   // [ntiles + 1]
+    int* s_ngt = nullptr;   // [ntiles] ghosts of the tile's own cells at the head of its S2 list
+    int* s2 = nullptr;      // S2 cell ids, tile by tile
+    int smax = 0;           // max over tiles of |T| + |H1| + |S2| (staged states)
+    int fsmax = 0;          // max over tiles of |T| + |H1| + ghosts of T (staged face states)
+    int h1max = 0, s2max = 0;
+    long long nh1 = 0, ns2 = 0;
+};
+
+// Builds the tables from the neighbour planes cnb ([nslot][NcPad], neighbour |
+// twin << 29 or SLOT_EMPTY) of the Nint interior cells.
+void build_halo_tables(int Nint, int NcPad, int nslot, const int* cnb, int TC, HaloTables& out, hipStream_t st);
+void free_halo_tables(HaloTables& t);
+
+// Bytes of dynamic LDS the single-kernel step needs (block of 256 lanes):
+// staged states and face states (FP64: 32 B each, FP32-mixed: 16 B), the two
+// id lists and the tile-minimum slots (24 B, 8-aligned).
+inline size_t fused_lds_bytes(bool mixed, int maxdeg, int smax, int fsmax, int h1max, int s2max) {
+    const size_t per = mixed ? 16 : 32;
+    size_t b = per * (size_t)smax + per * (size_t)maxdeg * fsmax;
+    b += 4 * (size_t)(((h1max + 3) & ~3) + ((s2max + 3) & ~3));
+    b += 32;
+    return b;
+}
+
+}  // namespace hcfd

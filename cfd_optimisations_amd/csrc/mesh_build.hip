@@ -632,6 +632,16 @@ void sort_pairs(unsigned long long* keys, int* vals, int n, hipStream_t st, Scra
     if (vb.Current() != vals) HIPCHK(hipMemcpyAsync(vals, vb.Current(), sizeof(int) * n, hipMemcpyDeviceToDevice, st));
 }
 
+// Counts interior cells whose 1/V is not a positive finite number (an inverted
+// or degenerate cell: the method's CFL bound and residual scaling need V > 0).
+__global__ void k_check_iv(int Nc, const unsigned char* __restrict__ ghost, const double* __restrict__ iv,
+                           int* __restrict__ nbad) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Nc || ghost[i]) return;
+    const double x = iv[i];
+    if (!(x > 0.0) || !(x < INFINITY)) atomicAdd(nbad, 1);
+}
+
 __global__ void k_validate(int Nf, int Nc, const int2* __restrict__ fc, const int* __restrict__ bcid,
                            const unsigned char* __restrict__ ghost, const int* __restrict__ deg, int* __restrict__ err) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -741,6 +751,13 @@ void build_device_mesh(const BuildInput& in, DeviceMesh& out, hipStream_t st) {
     k_geom_face<<<nblk(Nf), 256, 0, st>>>(Nf, d_fc, d_fn, d_nodes, d_xf, d_xc, d_bcid, d_rec);
     double* d_iv_old = sc.get<double>(Nc);
     k_geom_iv<<<nblk(Nc), 256, 0, st>>>(Nc, d_off, d_cf, d_fc, d_rec, d_ghost, d_iv_old);
+    {
+        int* d_bad = sc.get<int>(1);
+        HIPCHK(hipMemsetAsync(d_bad, 0, sizeof(int), st));
+        k_check_iv<<<nblk(Nc), 256, 0, st>>>(Nc, d_ghost, d_iv_old, d_bad);
+        const int nbad = read_int(d_bad, st);
+        if (nbad) fail(HCFD_ERR_MESH, std::to_string(nbad) + " interior cell(s) with non-positive or non-finite volume");
+    }
 
     // ---- interior cell order (RCM) ----------------------------------------------
     std::vector<unsigned char> ghost_h(in.ghost, in.ghost + Nc);

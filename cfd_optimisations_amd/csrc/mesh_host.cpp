@@ -85,14 +85,18 @@ std::unique_ptr<hcfd_mesh> structured_triangles(int nx, int ny, NodeFn node_xy, 
     m->Nn = NX * NY;
     m->nodes.resize(2 * (size_t)m->Nn);
     SplitMix64 rng(seed);
+    // Interior nodes move by up to +-jitter grid spacings in the LOGICAL (i, j)
+    // coordinates before the node map; the maps have a positive Jacobian, so the
+    // jittered cells keep their orientation even where the grid is sheared.
     for (int j = 0; j < NY; j++)
         for (int i = 0; i < NX; i++) {
-            double x, y, hx, hy;
-            node_xy(i, j, x, y, hx, hy);
+            double s = i, t = j;
             if (i > 0 && i < nx && j > 0 && j < ny && jitter > 0) {
-                x += (2.0 * rng.uniform() - 1.0) * jitter * hx;
-                y += (2.0 * rng.uniform() - 1.0) * jitter * hy;
+                s += (2.0 * rng.uniform() - 1.0) * jitter;
+                t += (2.0 * rng.uniform() - 1.0) * jitter;
             }
+            double x, y;
+            node_xy(s, t, x, y);
             m->nodes[2 * ((size_t)j * NX + i)] = x;
             m->nodes[2 * ((size_t)j * NX + i) + 1] = y;
         }
@@ -351,11 +355,9 @@ int hcfd_meshgen_riemann(int nx, int ny, double jitter, unsigned long long seed,
     return guard([&] {
         if (!out) fail(HCFD_ERR_ARG, "null out");
         static const char* names[4] = {"bottom", "right", "top", "left"};
-        auto fn = [&](int i, int j, double& x, double& y, double& hx, double& hy) {
-            x = (double)i / nx;
-            y = (double)j / ny;
-            hx = 1.0 / nx;
-            hy = 1.0 / ny;
+        auto fn = [&](double i, double j, double& x, double& y) {
+            x = i / nx;
+            y = j / ny;
         };
         *out = structured_triangles(nx, ny, fn, jitter, seed, shuffle, names).release();
     });
@@ -370,12 +372,10 @@ int hcfd_meshgen_ramp(int nx, int ny, double length, double height, double x_ram
         const double ta = std::tan(angle_deg * M_PI / 180.0);
         if ((length - x_ramp) * ta >= 0.95 * height) fail(HCFD_ERR_ARG, "ramp closes the channel");
         static const char* names[4] = {"lower", "outlet", "upper", "inlet"};
-        auto fn = [&](int i, int j, double& x, double& y, double& hx, double& hy) {
+        auto fn = [&](double i, double j, double& x, double& y) {
             x = length * i / nx;
             const double yb = x > x_ramp ? (x - x_ramp) * ta : 0.0;
             y = yb + (height - yb) * j / ny;
-            hx = length / nx;
-            hy = (height - yb) / ny;
         };
         *out = structured_triangles(nx, ny, fn, jitter, seed, shuffle, names).release();
     });

@@ -35,6 +35,15 @@ struct PassArgs {
     const BCEntry* bctab = nullptr;
     const int* ghost_bc = nullptr;
     int* fault = nullptr;               // [ninst]
+    // single-kernel step (halo tiles, see halo.hpp)
+    const int* h1_off = nullptr;
+    const int* h1 = nullptr;
+    const int* s_off = nullptr;
+    const int* s_ngt = nullptr;
+    const int* s2 = nullptr;
+    int smax = 0, fsmax = 0, h1max = 0, s2max = 0;
+    double* tacc = nullptr;             // [ninst] simulated time, advanced by dt (CFL)
+    double* lastdt = nullptr;           // [ninst]
 };
 
 // Variant selectors.
@@ -49,8 +58,14 @@ struct PassVariant {
 // Launch the passes with `grid` x `block` threads on `st`.
 hipError_t launch_pass_a(const PassVariant& v, const PassArgs& a, int grid, int block, hipStream_t st);
 hipError_t launch_pass_b(const PassVariant& v, const PassArgs& a, int grid, int block, hipStream_t st);
-// Per-instance CFL reduction: dt[b] = cfl * min(...); t[b] += dt[b]; last[b] = dt[b].
-hipError_t launch_dt_reduce(const PassArgs& a, double cfl, double* dt, double* t, double* last, hipStream_t st);
+// Per-instance CFL reduction: dt[b] = cfl * min(...); with advance, also
+// t[b] += dt[b] and last[b] = dt[b] (the single-kernel step advances them itself).
+hipError_t launch_dt_reduce(const PassArgs& a, double cfl, double* dt, double* t, double* last, hipStream_t st,
+                            bool advance = true);
+// Single-kernel step over halo tiles of 256 cells, block 256, `lds` bytes of dynamic LDS.
+hipError_t launch_fused_step(const PassVariant& v, const PassArgs& a, int grid, size_t lds, hipStream_t st);
+// CFL bound of the current state (tile partials), block 256.
+hipError_t launch_cfl_bound(const PassVariant& v, const PassArgs& a, int grid, hipStream_t st);
 bool pass_variant_supported(const PassVariant& v);
 int pass_degree_bucket(int maxdeg);
 

@@ -51,20 +51,35 @@ hipError_t launch_pass_b(const PassVariant& v, const PassArgs& a, int grid, int 
 
 // Per-instance CFL reduction over the tile partials of pass A (block b <-> instance b).
 __global__ __launch_bounds__(1024) void k_dt_reduce(PassArgs a, double cfl, double* __restrict__ dt,
-                                                    double* __restrict__ t, double* __restrict__ last) {
+                                                    double* __restrict__ t, double* __restrict__ last, int advance) {
     __shared__ double sh[1024 / WAVE];
     const int b = blockIdx.x;
     const int c0 = a.inst_off[b], c1 = a.inst_off[b + 1];
     double m = INFINITY;
     if (c1 > c0) {
         const int t0 = c0 / a.tile_cells, t1 = (c1 - 1) / a.tile_cells;
-        for (int tile = t0 + (int)threadIdx.x; tile <= t1; tile += blockDim.x) {
-            const int f0 = tile * a.tile_cells;
-            const int l0 = min(f0 + a.tile_cells, a.Nint) - 1;
-            const int bf = a.ninst > 1 ? instance_of(a.inst_off, a.ninst, f0) : 0;
-            const int bl = a.ninst > 1 ? instance_of(a.inst_off, a.ninst, l0) : 0;
-            if (bf == b) m = fmin(m, a.pmin_first[tile]);
-            else if (bl == b) m = fmin(m, a.pmin_last[tile]);
+        if (a.ninst == 1) {
+            // one instance: every partial is its tile's first; four loads in flight per lane
+            double m1 = INFINITY, m2 = INFINITY, m3 = INFINITY;
+            int tile = t0 + (int)threadIdx.x;
+            const int s = blockDim.x;
+            for (; tile + 3 * s <= t1; tile += 4 * s) {
+                m = fmin(m, a.pmin_first[tile]);
+                m1 = fmin(m1, a.pmin_first[tile + s]);
+                m2 = fmin(m2, a.pmin_first[tile + 2 * s]);
+                m3 = fmin(m3, a.pmin_first[tile + 3 * s]);
+            }
+            for (; tile <= t1; tile += s) m = fmin(m, a.pmin_first[tile]);
+            m = fmin(fmin(m, m1), fmin(m2, m3));
+        } else {
+            for (int tile = t0 + (int)threadIdx.x; tile <= t1; tile += blockDim.x) {
+                const int f0 = tile * a.tile_cells;
+                const int l0 = min(f0 + a.tile_cells, a.Nint) - 1;
+                const int bf = instance_of(a.inst_off, a.ninst, f0);
+                const int bl = instance_of(a.inst_off, a.ninst, l0);
+                if (bf == b) m = fmin(m, a.pmin_first[tile]);
+                else if (bl == b) m = fmin(m, a.pmin_last[tile]);
+            }
         }
     }
     m = block_min(m, sh);
@@ -72,14 +87,17 @@ __global__ __launch_bounds__(1024) void k_dt_reduce(PassArgs a, double cfl, doub
         m = fmin(m, __longlong_as_double((long long)a.dtacc[b]));
         const double h = cfl * m;
         dt[b] = h;
-        t[b] += h;
-        last[b] = h;
+        if (advance) {
+            t[b] += h;
+            last[b] = h;
+        }
         a.dtacc[b] = dbits(INFINITY);
     }
 }
 
-hipError_t launch_dt_reduce(const PassArgs& a, double cfl, double* dt, double* t, double* last, hipStream_t st) {
-    hipLaunchKernelGGL(k_dt_reduce, dim3(a.ninst), dim3(1024), 0, st, a, cfl, dt, t, last);
+hipError_t launch_dt_reduce(const PassArgs& a, double cfl, double* dt, double* t, double* last, hipStream_t st,
+                            bool advance) {
+    hipLaunchKernelGGL(k_dt_reduce, dim3(a.ninst), dim3(1024), 0, st, a, cfl, dt, t, last, advance ? 1 : 0);
     return hipGetLastError();
 }
 

@@ -46,7 +46,8 @@ class Case(ctypes.Structure):
 
 
 class Schedule(ctypes.Structure):
-    _fields_ = [("block", I), ("lanes_per_cell", I), ("aosoa_width", I), ("persistent", I), ("xcd_remap", I)]
+    _fields_ = [("block", I), ("lanes_per_cell", I), ("aosoa_width", I), ("persistent", I), ("xcd_remap", I),
+                ("fused", I)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -55,7 +56,8 @@ class Schedule(ctypes.Structure):
 class Info(ctypes.Structure):
     _fields_ = [("Nc", I), ("Nf", I), ("Nint", I), ("Nghost", I), ("ninst", I), ("maxdeg", I),
                 ("precision", I), ("flags", I), ("bandwidth_in", I), ("bandwidth_out", I), ("ncolours", I),
-                ("device_bytes", ctypes.c_longlong), ("sched", Schedule)]
+                ("device_bytes", ctypes.c_longlong), ("sched", Schedule), ("fused_available", I),
+                ("fused_lds_bytes", I), ("halo_max_cells", I)]
 
 
 _lib = None
@@ -422,8 +424,8 @@ class Context:
         check(load().hcfd_get_schedule(self.h, ctypes.byref(s)))
         return s.as_dict()
 
-    def set_schedule(self, block=256, lanes_per_cell=1, aosoa_width=1, persistent=0, xcd_remap=1):
-        s = Schedule(block, lanes_per_cell, aosoa_width, persistent, xcd_remap)
+    def set_schedule(self, block=256, lanes_per_cell=1, aosoa_width=1, persistent=0, xcd_remap=1, fused=0):
+        s = Schedule(block, lanes_per_cell, aosoa_width, persistent, xcd_remap, fused)
         check(load().hcfd_set_schedule(self.h, ctypes.byref(s)))
 
     def autotune(self, steps: int = 10, cfl: float = 0.0, cache: Optional[str] = None) -> dict:

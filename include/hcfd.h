@@ -97,6 +97,9 @@ typedef struct hcfd_schedule {
     int aosoa_width;     /* cell-state tile width: 1 (AoS), 8, 64 */
     int persistent;      /* 0: one tile per block; >0: blocks per CU of a persistent grid */
     int xcd_remap;       /* 1: map block ids so each XCD walks a contiguous cell range */
+    int fused;           /* 1: one kernel per step over tiles of 256 cells that recompute a one-cell
+                            halo ring (face states stay in LDS); needs block 256, lanes_per_cell 1
+                            and hcfd_info.fused_available */
 } hcfd_schedule;
 
 typedef struct hcfd_info {
@@ -107,6 +110,9 @@ typedef struct hcfd_info {
     int ncolours;        /* face colours of the conflict-free colouring */
     long long device_bytes;
     hcfd_schedule sched;
+    int fused_available;  /* the halo tiles of this mesh fit in LDS (see hcfd_schedule.fused) */
+    int fused_lds_bytes;  /* LDS per block of the single-kernel step */
+    int halo_max_cells;   /* most cells one tile stages: its 256 + halo ring + their neighbours */
 } hcfd_info;
 
 typedef struct hcfd_ctx hcfd_ctx;
@@ -196,11 +202,12 @@ int hcfd_get_last_dt(hcfd_ctx* ctx, double* dt);
 /* 1 if any step since the last call produced rho <= 0, p <= 0 or NaN. */
 int hcfd_get_fault(hcfd_ctx* ctx, int* fault);
 
-/* Runs nsteps fused steps on the context's stream with a HIP event pair
- * around every launch.  ms_out[0] = first to last event (wall of the nsteps
- * steps); ms_out[1] = mean pass A (gradient + limiter (+ CFL bound));
- * ms_out[2] = mean pass B (flux + update + ghost BCs); ms_out[3] = mean CFL
- * reduction (0 for fixed dt). */
+/* Runs nsteps steps on the context's stream with a HIP event pair around
+ * every launch.  ms_out[0] = first to last event (wall of the nsteps steps);
+ * ms_out[1] = mean pass A (gradient + limiter (+ CFL bound)), or mean of the
+ * single-kernel step when the schedule is fused; ms_out[2] = mean pass B
+ * (flux + update + ghost BCs), 0 when fused; ms_out[3] = mean CFL reduction
+ * (0 for fixed dt). */
 int hcfd_time_steps(hcfd_ctx* ctx, int nsteps, double dt, double cfl, float* ms_out);
 
 /* Device copy ceiling: a plain 16-byte-per-lane copy kernel over `bytes` bytes

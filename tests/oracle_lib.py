@@ -17,6 +17,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(ROOT, "oracle", "_build", "libhcfd_oracle.so")
 REF_SO = os.path.join(ROOT, "oracle", "_ref", "libhcfd_ref.so")
 REF_NOFMA_SO = os.path.join(ROOT, "oracle", "_ref", "libhcfd_ref_nofma.so")
+REF_VEC_SO = os.path.join(ROOT, "oracle", "_ref", "libhcfd_ref_vec.so")
 
 P = ctypes.c_void_p
 D = ctypes.c_double
@@ -168,6 +169,48 @@ class OracleDomain:
 
 def ref_available(path: str = REF_SO) -> bool:
     return os.path.exists(path)
+
+
+class VecRefCase:
+    """The reference's VEC layout (oracle/_ref, AVX2 build), driven through
+    ref_vec_driver.cpp: its fastest valid CPU layout, used as the CPU baseline.
+    The state is [4][Nc]; ghost 1/V is zeroed at creation as Euler does."""
+
+    def __init__(self, hmsh_path: str, gamma: float, bcmap_by_coltag, qbc, so: str = REF_VEC_SO):
+        lib = ctypes.CDLL(so)
+        lib.vref_create.restype = P
+        lib.vref_create.argtypes = [ctypes.c_char_p, D, P, P]
+        lib.vref_destroy.argtypes = [P]
+        lib.vref_run.argtypes = [P, P, I, D]
+        lib.vref_veclen.restype = I
+        lib.vref_prims.argtypes = [P, D, P, P]
+        self.lib, self.gamma = lib, gamma
+        self._bcmap = np.ascontiguousarray(bcmap_by_coltag, dtype=np.int32)
+        self._qbc = np.ascontiguousarray(qbc, dtype=np.float64)
+        self.h = lib.vref_create(hmsh_path.encode(), gamma, _ptr(self._bcmap), _ptr(self._qbc))
+        self.veclen = lib.vref_veclen()
+
+    def run(self, U, nsteps: int, dt: float):
+        U = np.array(U, dtype=np.float64, copy=True, order="C")
+        self.lib.vref_run(self.h, _ptr(U), nsteps, dt)
+        return U
+
+    def prims(self, U):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        Q = np.zeros_like(U)
+        self.lib.vref_prims(self.h, self.gamma, _ptr(U), _ptr(Q))
+        return Q
+
+    def close(self):
+        if self.h:
+            self.lib.vref_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class RefCase:

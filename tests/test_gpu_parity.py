@@ -1,9 +1,10 @@
 """GPU parity: the HIP path through the C ABI against the CPU oracle.
 
-FP64 with HCFD_FLAG_EXACT must be bit-identical to the oracle (which is itself
-bit-identical to the reference's Base sources built with -ffp-contract=off);
-the default FP64 path (limited gradient stored as phi*G) must stay within
-1e-12 normwise relative per variable; the mixed path within its stated bound.
+Every FP64 schedule (two passes per step, or one kernel per step over halo
+tiles) must be bit-identical to the oracle, which is itself bit-identical to
+the reference's Base sources built with -ffp-contract=off
+of 1e-12 normwise relative per variable is checked as well.  The mixed path
+must stay within its stated bound of the FP64 path.
 """
 import numpy as np
 import pytest
@@ -18,6 +19,10 @@ from conftest import GOLDEN
 pytestmark = pytest.mark.gpu
 
 FP64_TOL = 1e-12   # north-star FP64 bound (normwise relative, per variable)
+
+# the two step paths: two passes per step, and one kernel per step over halo tiles
+TWO_PASS = dict(block=256, lanes_per_cell=1, aosoa_width=1, persistent=0, xcd_remap=1, fused=0)
+FUSED = dict(TWO_PASS, fused=1)
 
 
 def relerr(a, b):
@@ -113,21 +118,46 @@ def test_wedge_reference_case_exact(wedge_mesh):
         assert ctx.fault() == 0
 
 
-def test_wedge_reference_case_fast(wedge_mesh):
+@pytest.mark.parametrize("sched", [TWO_PASS, FUSED], ids=["two_pass", "fused"])
+def test_wedge_reference_case_schedules(wedge_mesh, sched):
+    """src/Test.cpp's case through both step paths: bit-identical to the oracle."""
     case = C.reference_wedge_case(wedge_mesh)
     od, U0 = oracle_for(case)
     Uo, _ = od.run(U0, 1000, dt=1e-3)
     with C.make_context(case, L.FP64, 0) as ctx:
+        if sched["fused"] and not ctx.info().fused_available:
+            pytest.skip("wedge tiles do not fit the single-kernel step")
+        ctx.set_schedule(**sched)
         ctx.step(1000, dt=1e-3)
-        assert relerr(ctx.get_state(), Uo).max() < FP64_TOL
+        assert np.array_equal(ctx.get_state(), Uo)
+
+
+def test_fixed_dt_after_cfl_steps():
+    """Fixed dt, then CFL steps, then the same fixed dt again: the CFL reduction
+    overwrites the device dt, which must be reloaded."""
+    case = C.riemann_case(30, 20, seed=4)
+    od, U0 = oracle_for(case)
+    U1, _ = od.run(U0, 5, dt=2e-4)
+    U2, _ = od.run(U1, 5, cfl=0.5)
+    U3, _ = od.run(U2, 5, dt=2e-4)
+    for sched in (TWO_PASS, FUSED):
+        with C.make_context(case, L.FP64, 0) as ctx:
+            ctx.set_schedule(**sched)
+            ctx.step(5, dt=2e-4)
+            ctx.step(5, cfl=0.5)
+            ctx.step(5, dt=2e-4)
+            assert np.array_equal(ctx.get_state(), U3), sched
 
 
 @pytest.mark.parametrize("shuffle", [True, False])
-def test_riemann_cfl_exact(shuffle):
+@pytest.mark.parametrize("sched", [TWO_PASS, FUSED], ids=["two_pass", "fused"])
+def test_riemann_cfl_exact(shuffle, sched):
     case = C.riemann_case(40, 20, seed=0, shuffle=shuffle)
     od, U0 = oracle_for(case)
     Uo, to = od.run(U0, 60, cfl=0.5)
-    with C.make_context(case, L.FP64, L.FLAG_EXACT) as ctx:
+    with C.make_context(case, L.FP64, L.FLAG_EXACT | L.FLAG_ORDER_SFC) as ctx:
+        assert ctx.info().fused_available
+        ctx.set_schedule(**sched)
         t = ctx.step(60, cfl=0.5)
         assert t == to
         assert np.array_equal(ctx.get_state(), Uo)
@@ -141,6 +171,19 @@ def test_riemann_cfl_fast():
         t = ctx.step(100, cfl=0.5)
         assert abs(t - to) / to < 1e-12
         assert relerr(ctx.get_state(), Uo).max() < FP64_TOL
+
+
+def test_inverted_cell_rejected():
+    """A mesh with a cell of negative volume is refused at creation (the method's
+    residual scaling and CFL bound need V > 0)."""
+    case = C.riemann_case(6, 4, seed=0, shuffle=False, jitter=0.0)
+    hm = case.hmesh()
+    nodes = np.array(hm.nodes, dtype=np.float64).reshape(-1, 2)
+    nodes[8] = nodes[8] + np.array([0.5, 0.0])   # drag node (1, 1) across its neighbours
+    import dataclasses
+    bad = dataclasses.replace(hm, nodes=nodes)
+    with pytest.raises(L.HcfdError, match="volume"):
+        L.Context.create(bad, L.FP64, 0)
 
 
 def test_permutation_and_colouring():
@@ -167,7 +210,10 @@ def test_schedules_bitwise_identical():
                   dict(block=128, aosoa_width=64, persistent=0, xcd_remap=1),
                   dict(block=256, aosoa_width=1, persistent=8, xcd_remap=1),
                   dict(block=256, lanes_per_cell=4, aosoa_width=1, persistent=0, xcd_remap=1),
-                  dict(block=64, lanes_per_cell=4, aosoa_width=64, persistent=8, xcd_remap=0)):
+                  dict(block=64, lanes_per_cell=4, aosoa_width=64, persistent=8, xcd_remap=0),
+                  dict(block=256, aosoa_width=1, persistent=0, xcd_remap=1, fused=1),
+                  dict(block=256, aosoa_width=8, persistent=2, xcd_remap=0, fused=1),
+                  dict(block=256, aosoa_width=64, persistent=0, xcd_remap=1, fused=1)):
         with C.make_context(case, L.FP64, 0) as ctx:
             if sched:
                 ctx.set_schedule(**sched)
@@ -179,9 +225,12 @@ def test_schedules_bitwise_identical():
             assert np.array_equal(U, ref), sched
 
 
-def test_graph_replay_matches_eager():
+@pytest.mark.parametrize("sched", [TWO_PASS, FUSED], ids=["two_pass", "fused"])
+def test_graph_replay_matches_eager(sched):
     case = C.riemann_case(40, 40, seed=2)
     with C.make_context(case, L.FP64, 0) as a, C.make_context(case, L.FP64, L.FLAG_GRAPH) as b:
+        a.set_schedule(**sched)
+        b.set_schedule(**sched)
         a.step(21, cfl=0.5)
         b.step(21, cfl=0.5)
         assert np.array_equal(a.get_state(), b.get_state())
@@ -190,10 +239,12 @@ def test_graph_replay_matches_eager():
         assert np.array_equal(a.get_state(), b.get_state())
 
 
-def test_batch_matches_single_exact():
+@pytest.mark.parametrize("sched", [TWO_PASS, FUSED], ids=["two_pass", "fused"])
+def test_batch_matches_single_exact(sched):
     cs = [C.ramp_case(24, 12, 2.0, 10.0, seed=1), C.ramp_case(40, 20, 3.0, 20.0, seed=2),
           C.ramp_case(16, 8, 1.8, 5.0, seed=3)]
-    with C.make_batch(cs, L.FP64, L.FLAG_EXACT) as b:
+    with C.make_batch(cs, L.FP64, L.FLAG_EXACT | L.FLAG_ORDER_SFC) as b:
+        b.set_schedule(**sched)
         b.step(40, cfl=0.5)
         tb, dtb, fb = b.status()
         for k, c in enumerate(cs):
@@ -208,34 +259,60 @@ def _physical(od, U, ghost):
     return bool(np.isfinite(U).all() and (Q[0][~ghost] > 0).all() and (Q[3][~ghost] > 0).all())
 
 
-def test_batch_faulting_instance_is_isolated():
-    """A steep wedge (Mach 3.3, 29 deg) goes non-physical at step 9 under the
-    reference method itself (the oracle does too); in a batch only that instance
-    is flagged, it matches the oracle bit for bit up to the fault, and the other
-    instances stay bit-exact."""
-    calm1 = C.ramp_case(40, 20, 2.0, 10.0, seed=5)
-    steep = C.ramp_case(48, 24, 3.3, 29.0, seed=5)
-    calm2 = C.ramp_case(32, 16, 1.8, 5.0, seed=5)
-    cs = [calm1, steep, calm2]
-    with C.make_batch(cs, L.FP64, 0) as b:
-        b.step(8, cfl=0.5)
-        od, U0 = oracle_for(steep)
-        U8, _ = od.run(U0, 8, cfl=0.5)
-        assert _physical(od, U8, steep.hmesh().ghost_mask())
-        assert np.array_equal(b.inst_state(1), U8)
+def _double_rarefaction(case, od, speed=2.0, p=0.4):
+    """Gas at rest density 1, pressure p, moving apart at +-speed from x = 0.5:
+    the near-vacuum between the two rarefactions drives the reference method's
+    conserved-variable MUSCL states to negative pressure within a few steps."""
+    g = case.gamma
+    hm = case.hmesh()
+    ghost = hm.ghost_mask()
+    left = C.cell_centroids(hm)[:, 0] < 0.5
+    U = od.initialise(case.qic)
+    for mask, u in ((~ghost & left, -speed), (~ghost & ~left, speed)):
+        U[:, mask] = np.array([1.0, u, 0.0, p / (g - 1) + 0.5 * u * u])[:, None]
+    return od.bc_cells(U)
+
+
+@pytest.mark.parametrize("sched", [TWO_PASS, FUSED], ids=["two_pass", "fused"])
+def test_batch_faulting_instance_is_isolated(sched):
+    """An instance started as a strong double rarefaction goes non-physical at
+    step 8 under the reference method itself (the oracle does too); in a batch
+    only that instance is flagged, it matches the oracle bit for bit until the
+    fault, and the other instances stay bit-exact."""
+    cs = [C.ramp_case(40, 20, 2.0, 10.0, seed=5), C.ramp_case(48, 24, 2.0, 10.0, seed=5),
+          C.ramp_case(32, 16, 1.8, 5.0, seed=5)]
+    od, _ = oracle_for(cs[1])
+    U0 = _double_rarefaction(cs[1], od)
+    ghost = cs[1].hmesh().ghost_mask()
+    with C.make_batch(cs, L.FP64, L.FLAG_ORDER_SFC) as b:
+        b.set_schedule(**sched)
+        b.set_inst_state(1, U0)
+        b.step(7, cfl=0.5)
+        U7, _ = od.run(U0, 7, cfl=0.5)
+        assert _physical(od, U7, ghost)
+        assert np.array_equal(b.inst_state(1), U7)
         assert not b.status()[2].any()
-        b.step(32, cfl=0.5)
+        b.step(33, cfl=0.5)
         _, _, f = b.status()
         assert list(f) == [0, 1, 0]
-        U40, _ = od.run(U0, 40, cfl=0.5)
-        assert not _physical(od, U40, steep.hmesh().ghost_mask())
+        U8, _ = od.run(U0, 8, cfl=0.5)
+        assert not _physical(od, U8, ghost)
         for k in (0, 2):
             odk, Uk0 = oracle_for(cs[k])
             Uk, _ = odk.run(Uk0, 40, cfl=0.5)
             assert np.array_equal(b.inst_state(k), Uk), k
 
 
-MIXED_RHO_TOL = 1e-5   # north-star bound on the relative L2 density error vs FP64
+def test_mixed_fused_equals_two_pass():
+    """The FP32-mixed single-kernel step performs the two-pass arithmetic exactly."""
+    case = C.ramp_case(120, 60, 3.0, 20.0, seed=0)
+    with C.make_context(case, L.MIXED, L.FLAG_ORDER_SFC) as a, C.make_context(case, L.MIXED, L.FLAG_ORDER_SFC) as b:
+        a.set_schedule(**TWO_PASS)
+        b.set_schedule(**FUSED)
+        ta = a.step(100, cfl=0.5)
+        tb = b.step(100, cfl=0.5)
+        assert ta == tb
+        assert np.array_equal(a.get_state(), b.get_state())
 
 
 def test_mixed_precision_error_bound():

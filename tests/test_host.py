@@ -88,6 +88,23 @@ def test_ramp_generator_area():
         L.NativeMesh.ramp(10, 5, 1.0, 0.5, 0.2, 40.0, 0.2, 3, False)   # closes the channel
 
 
+@pytest.mark.parametrize("angle", [20.0, 29.0, 30.0])
+def test_ramp_generator_keeps_cells_valid(angle):
+    """Jitter is applied in logical grid coordinates before the node map, so even
+    on steep ramps (sheared grid lines in the converging channel) every triangle
+    keeps a positive orientation and every interior cell a positive volume."""
+    m = L.NativeMesh.ramp(48, 24, 1.0, 0.5, 0.2, angle, 0.2, 5, False).to_hmesh()
+    nodes = m.nodes.reshape(-1, 2)
+    interior = np.nonzero(~m.ghost_mask())[0]
+    tri = np.array([m.cn_con[m.cn_off[i]:m.cn_off[i + 1]] for i in interior])
+    p = nodes[tri]
+    area = 0.5 * ((p[:, 1, 0] - p[:, 0, 0]) * (p[:, 2, 1] - p[:, 0, 1])
+                  - (p[:, 2, 0] - p[:, 0, 0]) * (p[:, 1, 1] - p[:, 0, 1]))
+    assert (area > 0).all()
+    od = ol.OracleDomain(m)
+    assert (od.geometry()["iv"][interior] > 0).all()
+
+
 def test_hmsh_and_hsln_round_trip(tmp_path, wedge_mesh):
     p = str(tmp_path / "w.hmsh")
     write_hmsh(p, wedge_mesh)

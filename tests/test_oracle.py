@@ -80,6 +80,22 @@ def test_reference_build_reproduces_golden(wedge, wedge_path):
     assert np.array_equal(rc.prims(U), B1)
 
 
+@pytest.mark.skipif(not ol.ref_available(ol.REF_VEC_SO), reason="oracle/_ref not built (no /root/reference)")
+def test_reference_vec_build_reproduces_golden(wedge, wedge_path):
+    """The VEC layout (the CPU baseline), AVX2 build, reproduces the shipped
+    sols/VEC_1.hsln bit for bit and stays within 5e-14 of Base_1."""
+    m, od, qic, qbc, bct, bcmap = wedge
+    vc = ol.VecRefCase(wedge_path, G, bcmap, qbc)
+    assert vc.veclen == 4
+    U = vc.run(od.initialise(qic), 1000, 1e-3)
+    _, V1 = read_hsln(os.path.join(GOLDEN, "VEC_1.hsln"))
+    Q = vc.prims(U)
+    assert np.array_equal(Q, V1)
+    _, B1 = read_hsln(os.path.join(GOLDEN, "Base_1.hsln"))
+    err = max(np.linalg.norm(Q[v] - B1[v]) / np.linalg.norm(B1[v]) for v in range(4))
+    assert err < 5e-14
+
+
 def _compare_stages(m, path, bct_by_tag, qbc, qic, steps=40, dt=1e-3):
     bct = np.array([bct_by_tag[m.coltag[m.Ncol - m.NBCcol + b]] for b in range(m.NBCcol)])
     od = ol.OracleDomain(m, G, bct, qbc)
