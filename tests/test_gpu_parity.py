@@ -303,6 +303,9 @@ def test_batch_faulting_instance_is_isolated(sched):
             assert np.array_equal(b.inst_state(k), Uk), k
 
 
+MIXED_RHO_TOL = 1e-5   # north-star bound on the relative L2 density error vs FP64
+
+
 def test_mixed_fused_equals_two_pass():
     """The FP32-mixed single-kernel step performs the two-pass arithmetic exactly."""
     case = C.ramp_case(120, 60, 3.0, 20.0, seed=0)
