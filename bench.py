@@ -221,7 +221,7 @@ def secondary_c3(steps=None):
     out = []
     states = {}
     for prec in (L.FP64, L.MIXED):
-        ctx = C.make_context(case, prec, 0)
+        ctx = C.make_context(case, prec, L.FLAG_ORDER_SFC)
         inf = ctx.info()
         tm = ctx.time_steps(steps, cfl=w["cfl"])
         _, _, step_b = alg_bytes(inf.Nc, inf.Nf, prec == L.MIXED)

@@ -309,7 +309,10 @@ hcfd_ctx* create_ctx(int n, const hcfd_mesh_view* views, int precision, int flag
     {
         int lim = 0;
         HIPCHK(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
-        c->fused_ok = c->fused_lds <= (size_t)std::min(lim, 80 * 1024);
+        // compact tiles only: a tile whose halo exceeds its own size (as with the
+        // strip-shaped tiles of a reverse Cuthill-McKee order) recomputes more
+        // than it updates; the Hilbert order keeps halos near a quarter
+        c->fused_ok = c->fused_lds <= (size_t)std::min(lim, 80 * 1024) && c->halo.smax <= 2 * 256;
     }
     c->sched.fused = c->fused_ok ? 1 : 0;  // one kernel per step where the halo tiles fit
     HIPCHK(hipStreamSynchronize(c->st));

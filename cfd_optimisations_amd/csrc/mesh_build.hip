@@ -370,7 +370,7 @@ __global__ void k_sfc_keys(int Nc, const int* __restrict__ inst_off, int ninst,
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= Nc || ghost[i]) return;
     const int b = instance_of(inst_off, ninst, i);
-    const double4 bb = bbox[b];  // xmin, ymin, 1/(xmax-xmin), 1/(ymax-ymin)
+    const double4 bb = bbox[b];  // xmin, ymin, 1/extent, 1/extent (extent = the larger side)
     const double2 p = xc[i];
     double fx = (p.x - bb.x) * bb.z, fy = (p.y - bb.y) * bb.w;
     fx = fmin(fmax(fx, 0.0), 1.0);
@@ -849,10 +849,13 @@ void build_device_mesh(const BuildInput& in, DeviceMesh& out, hipStream_t st) {
                     x0 = std::min(x0, x); x1 = std::max(x1, x);
                     y0 = std::min(y0, y); y1 = std::max(y1, y);
                 }
+            // one scale for both axes: a run of consecutive curve indices then
+            // covers a square region (compact tiles) whatever the aspect ratio
+            const double ext = std::max(x1 - x0, y1 - y0);
             bb[4 * b] = x0;
             bb[4 * b + 1] = y0;
-            bb[4 * b + 2] = x1 > x0 ? 1.0 / (x1 - x0) : 1.0;
-            bb[4 * b + 3] = y1 > y0 ? 1.0 / (y1 - y0) : 1.0;
+            bb[4 * b + 2] = ext > 0 ? 1.0 / ext : 1.0;
+            bb[4 * b + 3] = ext > 0 ? 1.0 / ext : 1.0;
         }
         double* d_bb = sc.get<double>(bb.size());
         h2d(d_bb, bb.data(), bb.size(), st);

@@ -141,7 +141,7 @@ def test_fixed_dt_after_cfl_steps():
     U2, _ = od.run(U1, 5, cfl=0.5)
     U3, _ = od.run(U2, 5, dt=2e-4)
     for sched in (TWO_PASS, FUSED):
-        with C.make_context(case, L.FP64, 0) as ctx:
+        with C.make_context(case, L.FP64, L.FLAG_ORDER_SFC) as ctx:
             ctx.set_schedule(**sched)
             ctx.step(5, dt=2e-4)
             ctx.step(5, cfl=0.5)
@@ -214,7 +214,7 @@ def test_schedules_bitwise_identical():
                   dict(block=256, aosoa_width=1, persistent=0, xcd_remap=1, fused=1),
                   dict(block=256, aosoa_width=8, persistent=2, xcd_remap=0, fused=1),
                   dict(block=256, aosoa_width=64, persistent=0, xcd_remap=1, fused=1)):
-        with C.make_context(case, L.FP64, 0) as ctx:
+        with C.make_context(case, L.FP64, L.FLAG_ORDER_SFC) as ctx:
             if sched:
                 ctx.set_schedule(**sched)
             ctx.step(30, cfl=0.5)
@@ -228,7 +228,8 @@ def test_schedules_bitwise_identical():
 @pytest.mark.parametrize("sched", [TWO_PASS, FUSED], ids=["two_pass", "fused"])
 def test_graph_replay_matches_eager(sched):
     case = C.riemann_case(40, 40, seed=2)
-    with C.make_context(case, L.FP64, 0) as a, C.make_context(case, L.FP64, L.FLAG_GRAPH) as b:
+    with C.make_context(case, L.FP64, L.FLAG_ORDER_SFC) as a, \
+            C.make_context(case, L.FP64, L.FLAG_GRAPH | L.FLAG_ORDER_SFC) as b:
         a.set_schedule(**sched)
         b.set_schedule(**sched)
         a.step(21, cfl=0.5)
