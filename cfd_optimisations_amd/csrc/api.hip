@@ -83,8 +83,9 @@ struct hcfd_ctx {
     int nbctab = 0;
     int* d_old_of_new = nullptr;
     int* d_ghost_inst = nullptr;
-    double* U[2] = {nullptr, nullptr};
+    double* U[3] = {nullptr, nullptr, nullptr};  // states; the third once a multi-stage integrator is chosen
     int cur = 0;
+    int stages = 1;          // time integrator: 1 forward Euler, 2 SSP-RK2, 3 SSP-RK3
     double* FS = nullptr;    // FP64 face states, [deg_bucket][2][NcPad] double2 planes
     float* FS32 = nullptr;   // FP32-mixed face states, [deg_bucket][NcPad] float4
     float* cgeo32 = nullptr; // FP32 per-slot geometry planes (mixed path)
@@ -105,12 +106,20 @@ struct hcfd_ctx {
     hcfd_schedule sched{256, 1, 1, 0, 0, 0};
     // single-kernel step over halo tiles
     HaloTables halo;
-    bool fused_ok = false;
+    bool fused_ok = false;       // the single-kernel step can run on this mesh
+    bool fused_default = false;  // ... and is the default schedule (compact tiles)
     size_t fused_lds = 0;
     bool bound_valid = false;  // dt holds the CFL step of the current state (single-kernel path)
     double t_host = 0.0;
     double fixed_dt_loaded = NAN;
-    // captured two-step graph
+    // output slots: primitives on the device, their pinned host copy (+ the clock)
+    hipStream_t cst = nullptr;   // copy stream (default priority)
+    double* out_dev[2] = {nullptr, nullptr};
+    double* out_host[2] = {nullptr, nullptr};
+    hipEvent_t out_ready[2] = {nullptr, nullptr};
+    hipEvent_t out_copied[2] = {nullptr, nullptr};
+    bool out_pending[2] = {false, false};
+    // captured graph of nbuf steps (U returns to the same buffer)
     hipGraphExec_t gexec = nullptr;
     double graph_dt = NAN, graph_cfl = NAN;
     int graph_cur = -1;
@@ -127,7 +136,8 @@ void free_ptr(void* p) {
 
 std::string sched_key(const hcfd_schedule& s) {
     char b[96];
-    snprintf(b, sizeof b, "%d/%d/%d/%d/%d", s.block, s.lanes_per_cell, s.aosoa_width, s.persistent, s.xcd_remap);
+    snprintf(b, sizeof b, "%d/%d/%d/%d/%d/%d", s.block, s.lanes_per_cell, s.aosoa_width, s.persistent, s.xcd_remap,
+             s.fused);
     return b;
 }
 
@@ -303,18 +313,30 @@ hcfd_ctx* create_ctx(int n, const hcfd_mesh_view* views, int precision, int flag
     HIPCHK(hipMemsetAsync(c->bctab, 0, sizeof(BCEntry) * c->nbctab, c->st));
     c->sched.xcd_remap = 1;
     // halo tiles of the single-kernel step; used when their LDS footprint fits
-    build_halo_tables(c->Nint, c->NcPad, c->deg_bucket, c->m.cnb, 256, c->halo, c->st);
+    // LDS share of one block at the occupancy the registers allow: FP64 three
+    // blocks of four wavefronts per CU (two for octagon meshes), FP32-mixed four
+    {
+        const int blocks = precision == HCFD_MIXED ? 4 : (c->deg_bucket <= 4 ? 3 : 2);
+        const size_t budget = (size_t)160 * 1024 / blocks - 512;
+        build_halo_tables(c->Nint, c->NcPad, c->deg_bucket, c->m.cnb, 256, precision == HCFD_MIXED, budget, c->halo,
+                          c->st);
+    }
     c->fused_lds = fused_lds_bytes(precision == HCFD_MIXED, c->deg_bucket, c->halo.smax, c->halo.fsmax,
                                    c->halo.h1max, c->halo.s2max);
     {
         int lim = 0;
         HIPCHK(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
-        // compact tiles only: a tile whose halo exceeds its own size (as with the
-        // strip-shaped tiles of a reverse Cuthill-McKee order) recomputes more
-        // than it updates; the Hilbert order keeps halos near a quarter
-        c->fused_ok = c->fused_lds <= (size_t)std::min(lim, 80 * 1024) && c->halo.smax <= 2 * 256;
+        // runnable when the largest tile's halo fits in LDS with two blocks per CU
+        c->fused_ok = c->fused_lds <= (size_t)std::min(lim, 80 * 1024);
+        // default only in FP64 and for compact tiles (at most 5% of the tiles had to be
+        // split): strip-shaped tiles, as a reverse Cuthill-McKee order makes, recompute
+        // more halo than they update; Hilbert tiles keep the halo near a quarter.  In
+        // FP32-mixed the halo recompute costs about what the face-state round trip
+        // saves (unpacked FP32 issues at the FP64 rate), so the two passes stay the
+        // default there and the autotuner may still pick the single kernel
+        c->fused_default = precision != HCFD_MIXED && c->fused_ok && c->halo.split_tiles * 20 <= c->halo.ntiles;
     }
-    c->sched.fused = c->fused_ok ? 1 : 0;  // one kernel per step where the halo tiles fit
+    c->sched.fused = c->fused_default ? 1 : 0;
     HIPCHK(hipStreamSynchronize(c->st));
     return c.release();
 }
@@ -382,6 +404,8 @@ struct Launch {
     int grid = 0, block = 0;
 };
 
+int nbuf(const hcfd_ctx* c) { return c->U[2] ? 3 : 2; }
+
 // pass: 0 = pass A (tiles of block / lanes_per_cell cells), 1 = pass B (one cell per lane).
 Launch make_launch(hcfd_ctx* c, bool cfl, int cur, int pass) {
     Launch L;
@@ -398,7 +422,8 @@ Launch make_launch(hcfd_ctx* c, bool cfl, int cur, int pass) {
     a.NcPad = c->NcPad;
     a.ninst = c->ninst;
     a.tile_cells = fused ? 256 : s.block / L.v.lpc;
-    a.ntiles = (c->Nint + a.tile_cells - 1) / a.tile_cells;
+    a.ntiles = fused ? c->halo.ntiles : (c->Nint + a.tile_cells - 1) / a.tile_cells;
+    a.tile_start = fused ? c->halo.tile_start : nullptr;
     a.xcd_remap = s.xcd_remap;
     a.tiles_per_xcd = (a.ntiles + NXCD - 1) / NXCD;
     a.gamma = c->gamma;
@@ -409,7 +434,7 @@ Launch make_launch(hcfd_ctx* c, bool cfl, int cur, int pass) {
     a.iv = c->m.iv;
     a.iv32 = c->iv32;
     a.U = c->U[cur];
-    a.Unext = c->U[cur ^ 1];
+    a.Unext = c->U[(cur + 1) % nbuf(c)];
     a.FS = c->FS;
     a.FS32 = c->FS32;
     a.inst_off = c->m.inst_int_off;
@@ -456,36 +481,89 @@ void ensure_bound(hcfd_ctx* c, double cfl) {
     if (!c->sched.fused || !(cfl > 0) || c->bound_valid) return;
     Launch L = make_launch(c, true, c->cur, 2);
     HIPCHK(launch_cfl_bound(L.v, L.a, L.grid, c->st));
-    HIPCHK(launch_dt_reduce(L.a, cfl, c->dt, c->tdev, c->lastdt, c->st, false));
+    HIPCHK(launch_dt_reduce(L.a, cfl, c->dt, c->st));
     c->bound_valid = true;
 }
 
-// Enqueues one step reading U[cur]; returns nothing, caller flips cur.
-void enqueue_step(hcfd_ctx* c, int cur, double cfl, hipEvent_t* ev = nullptr) {
-    const bool use_cfl = cfl > 0;
-    if (c->sched.fused) {
-        // CFL: the kernel leaves the bound of U^{n+1} per tile; the reduction
-        // gives dt of the next step (the kernel advances the clocks by dt itself)
-        Launch F = make_launch(c, use_cfl, cur, 2);
-        if (ev) HIPCHK(hipEventRecord(ev[0], c->st));
-        HIPCHK(launch_fused_step(F.v, F.a, F.grid, c->fused_lds, c->st));
-        if (ev) HIPCHK(hipEventRecord(ev[1], c->st));
-        if (use_cfl) HIPCHK(launch_dt_reduce(F.a, cfl, c->dt, c->tdev, c->lastdt, c->st, false));
-        if (ev) HIPCHK(hipEventRecord(ev[2], c->st));
-        if (ev) HIPCHK(hipEventRecord(ev[3], c->st));
-        c->bound_valid = use_cfl;
-        return;
+// Buffers of one step from U[cur]: stage k reads U[in[k]] and writes U[out[k]];
+// stages after the first combine with U[cur] (U^n) and never overwrite it.
+// Forward Euler alternates between the buffers; SSP-RK uses all three.
+struct StagePlan {
+    int in[3] = {0, 0, 0}, out[3] = {0, 0, 0};
+    int result = 0;
+};
+StagePlan plan_step(const hcfd_ctx* c, int cur) {
+    StagePlan p;
+    const int nb = nbuf(c), x = (cur + 1) % nb, y = (cur + 2) % nb;
+    p.in[0] = cur;
+    p.out[0] = x;
+    p.result = x;
+    if (c->stages >= 2) {
+        p.in[1] = x;
+        p.out[1] = y;
+        p.result = y;
     }
-    c->bound_valid = false;
-    Launch L = make_launch(c, use_cfl, cur, 0);
-    Launch LB = make_launch(c, use_cfl, cur, 1);
+    if (c->stages == 3) {
+        p.in[2] = y;
+        p.out[2] = x;
+        p.result = x;
+    }
+    return p;
+}
+
+// Shu-Osher coefficients of stage k: U_k = RK_A U^n + RK_B (U_{k-1} + dt R(U_{k-1}))
+// (Gottlieb, Shu and Tadmor, SIAM Rev. 43 (2001) 89-112, eqs. 3.1-3.2).
+const double RK_A[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0.5, 0}, {0, 0.75, 1.0 / 3.0}};
+const double RK_B[4][3] = {{1, 1, 1}, {1, 0, 0}, {1, 0.5, 0}, {1, 0.25, 2.0 / 3.0}};
+
+void set_stage(PassArgs& a, const hcfd_ctx* c, const StagePlan& p, int k, int cur) {
+    a.U = c->U[p.in[k]];
+    a.Unext = c->U[p.out[k]];
+    a.U0 = k > 0 ? c->U[cur] : nullptr;
+    a.rk_a = RK_A[c->stages][k];
+    a.rk_b = RK_B[c->stages][k];
+}
+
+// Enqueues one step reading U[cur]; returns the buffer holding U^{n+1}.  Every
+// stage uses the dt of U^n.  Events (when given): ev[0] step start, ev[1] after
+// the first stage's pass A (or single kernel), ev[2] after its CFL reduction,
+// ev[3] step end.
+int enqueue_step(hcfd_ctx* c, int cur, double cfl, hipEvent_t* ev = nullptr) {
+    const bool use_cfl = cfl > 0;
+    const StagePlan p = plan_step(c, cur);
+    const int S = c->stages;
     if (ev) HIPCHK(hipEventRecord(ev[0], c->st));
-    HIPCHK(launch_pass_a(L.v, L.a, L.grid, L.block, c->st));
-    if (ev) HIPCHK(hipEventRecord(ev[1], c->st));
-    if (use_cfl) HIPCHK(launch_dt_reduce(L.a, cfl, c->dt, c->tdev, c->lastdt, c->st));
-    if (ev) HIPCHK(hipEventRecord(ev[2], c->st));
-    HIPCHK(launch_pass_b(LB.v, LB.a, LB.grid, LB.block, c->st));
+    for (int k = 0; k < S; k++) {
+        const bool last = k == S - 1;
+        if (c->sched.fused) {
+            // the first stage advances the clocks by dt (fixed, or reduced from
+            // the CFL bound of U^n); in CFL mode the last stage leaves the bound
+            // of U^{n+1} per tile and the reduction gives dt of the next step
+            Launch F = make_launch(c, use_cfl && last, cur, 2);
+            set_stage(F.a, c, p, k, cur);
+            F.a.advance = k == 0;
+            HIPCHK(launch_fused_step(F.v, F.a, F.grid, c->fused_lds, c->st));
+            if (ev && k == 0) HIPCHK(hipEventRecord(ev[1], c->st));
+            if (use_cfl && last) HIPCHK(launch_dt_reduce(F.a, cfl, c->dt, c->st));
+            if (ev && k == 0) HIPCHK(hipEventRecord(ev[2], c->st));
+        } else {
+            // CFL: pass A of the first stage takes the bound of U^n; pass B of
+            // the first stage advances the clocks by dt
+            Launch L = make_launch(c, use_cfl && k == 0, cur, 0);
+            Launch LB = make_launch(c, use_cfl && k == 0, cur, 1);
+            set_stage(L.a, c, p, k, cur);
+            set_stage(LB.a, c, p, k, cur);
+            LB.a.advance = k == 0;
+            HIPCHK(launch_pass_a(L.v, L.a, L.grid, L.block, c->st));
+            if (ev && k == 0) HIPCHK(hipEventRecord(ev[1], c->st));
+            if (use_cfl && k == 0) HIPCHK(launch_dt_reduce(L.a, cfl, c->dt, c->st));
+            if (ev && k == 0) HIPCHK(hipEventRecord(ev[2], c->st));
+            HIPCHK(launch_pass_b(LB.v, LB.a, LB.grid, LB.block, c->st));
+        }
+    }
     if (ev) HIPCHK(hipEventRecord(ev[3], c->st));
+    c->bound_valid = c->sched.fused && use_cfl;
+    return p.result;
 }
 
 void ensure_graph(hcfd_ctx* c, double dt, double cfl) {
@@ -496,8 +574,8 @@ void ensure_graph(hcfd_ctx* c, double dt, double cfl) {
     hipGraph_t g = nullptr;
     HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
     try {
-        enqueue_step(c, c->cur, cfl);
-        enqueue_step(c, c->cur ^ 1, cfl);
+        int k = c->cur;
+        for (int q = 0; q < nbuf(c); q++) k = enqueue_step(c, k, cfl);
     } catch (...) {
         hipGraph_t junk = nullptr;
         (void)hipStreamEndCapture(c->st, &junk);
@@ -522,17 +600,12 @@ void run_steps(hcfd_ctx* c, int nsteps, double dt, double cfl) {
     else c->fixed_dt_loaded = NAN;  // the CFL reduction overwrites dt
     ensure_bound(c, cfl);
     int done = 0;
-    if ((c->flags & HCFD_FLAG_GRAPH) && nsteps >= 2) {
+    if ((c->flags & HCFD_FLAG_GRAPH) && nsteps >= nbuf(c)) {
         ensure_graph(c, dt, cfl);
-        for (; done + 2 <= nsteps; done += 2) HIPCHK(hipGraphLaunch(c->gexec, c->st));
-        // two steps leave U at the same parity
+        // every step moves cur by the same amount: nbuf steps bring it back
+        for (; done + nbuf(c) <= nsteps; done += nbuf(c)) HIPCHK(hipGraphLaunch(c->gexec, c->st));
     }
-    for (; done < nsteps; done++) {
-        enqueue_step(c, c->cur, cfl);
-        c->cur ^= 1;
-    }
-    if (!(cfl > 0))
-        for (int s = 0; s < nsteps; s++) c->t_host += dt;
+    for (; done < nsteps; done++) c->cur = enqueue_step(c, c->cur, cfl);
 }
 
 double device_time(hcfd_ctx* c) {
@@ -615,10 +688,18 @@ void check_schedule(const hcfd_ctx* c, const hcfd_schedule& s) {
 
 hcfd_ctx::~hcfd_ctx() {
     if (st) (void)hipStreamSynchronize(st);
+    if (cst) (void)hipStreamSynchronize(cst);
+    for (int k = 0; k < 2; k++) {
+        if (out_host[k]) (void)hipHostFree(out_host[k]);
+        if (out_ready[k]) (void)hipEventDestroy(out_ready[k]);
+        if (out_copied[k]) (void)hipEventDestroy(out_copied[k]);
+        free_ptr(out_dev[k]);
+    }
+    if (cst) (void)hipStreamDestroy(cst);
     if (gexec) (void)hipGraphExecDestroy(gexec);
     free_device_mesh(m);
     free_halo_tables(halo);
-    for (void* p : {(void*)bctab, (void*)d_old_of_new, (void*)d_ghost_inst, (void*)U[0], (void*)U[1], (void*)FS,
+    for (void* p : {(void*)bctab, (void*)d_old_of_new, (void*)d_ghost_inst, (void*)U[0], (void*)U[1], (void*)U[2], (void*)FS,
                     (void*)FS32, (void*)cgeo32, (void*)iv32, (void*)pmin_first, (void*)pmin_last,
                     (void*)dtacc, (void*)dt, (void*)tdev, (void*)lastdt, (void*)fault, (void*)staging, (void*)sU,
                     (void*)sG, (void*)sPhi, (void*)sR, (void*)backup})
@@ -902,9 +983,67 @@ int hcfd_step(hcfd_ctx* c, int nsteps, double dt, double cfl, double* t_inout) {
     return guarded([&] {
         if (!c) fail(HCFD_ERR_ARG, "null");
         run_steps(c, nsteps, dt, cfl);
-        if (cfl > 0) c->t_host = device_time(c);
-        else HIPCHK(hipStreamSynchronize(c->st));
+        c->t_host = device_time(c);  // the kernels advance the clocks in both dt modes
         if (t_inout) *t_inout = c->t_host;
+    });
+}
+
+int hcfd_step_async(hcfd_ctx* c, int nsteps, double dt, double cfl) {
+    return guarded([&] {
+        if (!c) fail(HCFD_ERR_ARG, "null");
+        run_steps(c, nsteps, dt, cfl);
+    });
+}
+
+int hcfd_sync(hcfd_ctx* c, double* t) {
+    return guarded([&] {
+        if (!c) fail(HCFD_ERR_ARG, "null");
+        c->t_host = device_time(c);
+        if (t) *t = c->t_host;
+    });
+}
+
+int hcfd_output_begin(hcfd_ctx* c, int slot) {
+    return guarded([&] {
+        if (!c || slot < 0 || slot > 1) fail(HCFD_ERR_ARG, "slot must be 0 or 1");
+        if (c->out_pending[slot]) fail(HCFD_ERR_STATE, "output slot still pending: hcfd_output_wait first");
+        const size_t n = 4 * (size_t)c->Nc + 1;  // primitives [4][Nc], then the clock
+        if (!c->cst) HIPCHK(hipStreamCreateWithFlags(&c->cst, hipStreamNonBlocking));
+        if (!c->out_dev[slot]) {
+            c->out_dev[slot] = dmalloc<double>(n);
+            HIPCHK(hipEventCreateWithFlags(&c->out_ready[slot], hipEventDisableTiming));
+            HIPCHK(hipEventCreateWithFlags(&c->out_copied[slot], hipEventDisableTiming));
+            void* h = nullptr;
+            if (hipHostMalloc(&h, sizeof(double) * n, hipHostMallocDefault) != hipSuccess || !h) {
+                (void)hipGetLastError();
+                fail(HCFD_ERR_NOMEM, "pinned host allocation of an output slot failed");
+            }
+            c->out_host[slot] = static_cast<double*>(h);
+        }
+        double* d = c->out_dev[slot];
+        HIPCHK(launch_prims(c->sched.aosoa_width, c->Nc, c->d_old_of_new, c->U[c->cur], c->gamma, d, c->Nc, c->st));
+        HIPCHK(hipMemcpyAsync(d + 4 * (size_t)c->Nc, c->tdev, sizeof(double), hipMemcpyDeviceToDevice, c->st));
+        HIPCHK(hipEventRecord(c->out_ready[slot], c->st));
+        HIPCHK(hipStreamWaitEvent(c->cst, c->out_ready[slot], 0));
+        HIPCHK(hipMemcpyAsync(c->out_host[slot], d, sizeof(double) * n, hipMemcpyDeviceToHost, c->cst));
+        HIPCHK(hipEventRecord(c->out_copied[slot], c->cst));
+        c->out_pending[slot] = true;
+    });
+}
+
+int hcfd_output_wait(hcfd_ctx* c, int slot, double* const Q[4], double* t) {
+    return guarded([&] {
+        if (!c || slot < 0 || slot > 1) fail(HCFD_ERR_ARG, "slot must be 0 or 1");
+        if (!c->out_pending[slot]) fail(HCFD_ERR_STATE, "output slot was not begun");
+        c->out_pending[slot] = false;
+        HIPCHK(hipEventSynchronize(c->out_copied[slot]));
+        const double* h = c->out_host[slot];
+        if (Q)
+            for (int v = 0; v < NV; v++) {
+                if (!Q[v]) fail(HCFD_ERR_ARG, "null array");
+                std::memcpy(Q[v], h + (size_t)v * c->Nc, sizeof(double) * c->Nc);
+            }
+        if (t) *t = h[4 * (size_t)c->Nc];
     });
 }
 
@@ -939,12 +1078,7 @@ int hcfd_time_steps(hcfd_ctx* c, int nsteps, double dt, double cfl, float* ms_ou
         std::vector<hipEvent_t> ev(4 * nsteps);
         for (auto& e : ev) HIPCHK(hipEventCreate(&e));
         HIPCHK(hipStreamSynchronize(c->st));
-        for (int s = 0; s < nsteps; s++) {
-            enqueue_step(c, c->cur, cfl, &ev[4 * s]);
-            c->cur ^= 1;
-        }
-        if (!(cfl > 0))
-            for (int s = 0; s < nsteps; s++) c->t_host += dt;
+        for (int s = 0; s < nsteps; s++) c->cur = enqueue_step(c, c->cur, cfl, &ev[4 * s]);
         HIPCHK(hipEventSynchronize(ev[4 * nsteps - 1]));
         float ms = 0;
         HIPCHK(hipEventElapsedTime(&ms, ev[0], ev[4 * nsteps - 1]));
@@ -963,7 +1097,7 @@ int hcfd_time_steps(hcfd_ctx* c, int nsteps, double dt, double cfl, float* ms_ou
         ms_out[2] = (float)(sb / nsteps);
         ms_out[3] = (cfl > 0) ? (float)(sr / nsteps) : 0.0f;
         for (auto& e : ev) (void)hipEventDestroy(e);
-        if (cfl > 0) c->t_host = device_time(c);
+        c->t_host = device_time(c);
     });
 }
 
@@ -1010,11 +1144,33 @@ int hcfd_copy_bandwidth(long long bytes, int reps, double* gbs) {
 int hcfd_set_schedule(hcfd_ctx* c, const hcfd_schedule* s) {
     return guarded([&] {
         if (!c) fail(HCFD_ERR_ARG, "null");
-        hcfd_schedule n = s ? *s : hcfd_schedule{256, 1, 1, 0, 1, c->fused_ok ? 1 : 0};
+        hcfd_schedule n = s ? *s : hcfd_schedule{256, 1, 1, 0, 1, c->fused_default ? 1 : 0};
         check_schedule(c, n);
         change_width(c, n.aosoa_width);
         c->sched = n;
         drop_graph(c);
+    });
+}
+
+int hcfd_set_integrator(hcfd_ctx* c, int stages) {
+    return guarded([&] {
+        if (!c) fail(HCFD_ERR_ARG, "null");
+        if (stages < 1 || stages > 3) fail(HCFD_ERR_ARG, "stages must be 1, 2 or 3");
+        if (stages > 1 && !c->U[2]) {
+            const size_t n = 4 * (size_t)c->NcPad;  // as U[0], U[1]
+            c->U[2] = dmalloc<double>(n);
+            HIPCHK(hipMemsetAsync(c->U[2], 0, sizeof(double) * n, c->st));
+            HIPCHK(hipStreamSynchronize(c->st));
+        }
+        c->stages = stages;
+        drop_graph(c);
+    });
+}
+
+int hcfd_get_integrator(hcfd_ctx* c, int* stages) {
+    return guarded([&] {
+        if (!c || !stages) fail(HCFD_ERR_ARG, "null");
+        *stages = c->stages;
     });
 }
 

@@ -11,9 +11,11 @@
 //      side of their boundary faces (BoundaryConditions/Base.cpp:29-53);
 //   3. for every cell of T: Rusanov flux of each face from the two staged face
 //      states (Flux.cpp:52-109), R * 1/V, U^{n+1} = U^n + dt R
-//      (Euler.cpp:107-113), the ghost states of its boundary faces
-//      (BoundaryConditions/{Symmetry,Wall,Inlet,Outlet}.cpp) and, in CFL mode,
-//      the CFL bound of U^{n+1}: the tile partials give dt of the next step.
+//      (Euler.cpp:107-113) [+ the SSP-RK stage combination], the ghost states
+//      of its boundary faces (BoundaryConditions/{Symmetry,Wall,Inlet,Outlet}.cpp)
+//      and, in CFL mode, the CFL bound of U^{n+1}: the tile partials give dt of
+//      the next step.  A multi-stage step is one launch per stage; the CFL
+//      bound is taken by the last stage only.
 // H1 cells are computed by every tile that borders them, with the same
 // arithmetic as in their own tile, so each face state equals the one the
 // two-pass kernels produce (passes_impl.hpp) bit for bit; the results are those
@@ -249,13 +251,14 @@ struct TileMin {
 };
 
 template <bool MULTI>
-__device__ __forceinline__ void cfl_tile_lds(const PassArgs& a, int tile, int i, double dti, TileMin* tm) {
+__device__ __forceinline__ void cfl_tile_lds(const PassArgs& a, int tile, int tb, int tc, int i, double dti,
+                                             TileMin* tm) {
     const int lane = threadIdx.x & (WAVE - 1);
     const int nwaves = FUSED_TILE / WAVE;
     double vf = dti, vl = INFINITY;
     if constexpr (MULTI) {
-        const int c0 = tile * FUSED_TILE;
-        const int c1 = min(c0 + FUSED_TILE, a.Nint) - 1;
+        const int c0 = tb;
+        const int c1 = tb + tc - 1;
         const int bf = instance_of(a.inst_off, a.ninst, c0);
         const int bl = instance_of(a.inst_off, a.ninst, c1);
         vf = INFINITY;
@@ -308,20 +311,13 @@ k_fused_step(
         tmin->arrived = 0;
     }  // first read after the tile loop's first barrier
 
-    if constexpr (CFL) {
-        // this step's dt was reduced from the CFL bound of U^n: advance the clocks
-        if (blockIdx.x == 0)
-            for (int b = tid; b < a.ninst; b += blockDim.x) {
-                a.tacc[b] += a.dt[b];
-                a.lastdt[b] = a.dt[b];
-            }
-    }
+    advance_clocks(a);  // dt of this step: fixed, or reduced from the CFL bound of U^n
     int tfirst, step, tend;
     tile_walk(a, tfirst, step, tend);
     for (int tile = tfirst; tile < tend; tile += step) {
         TileLists L;
-        L.tb = tile * FUSED_TILE;
-        L.tc = min(FUSED_TILE, a.Nint - L.tb);
+        L.tb = a.tile_start[tile];
+        L.tc = a.tile_start[tile + 1] - L.tb;
         const int h0 = a.h1_off[tile], s0 = a.s_off[tile];
         L.nh1 = a.h1_off[tile + 1] - h0;
         L.ns2 = a.s_off[tile + 1] - s0;
@@ -417,6 +413,7 @@ k_fused_step(
 #pragma unroll
                 for (int v = 0; v < NV; v++) unew[v] = uown[v] + dt * (K[v].value() * (double)ci.iv);
             }
+            if (a.U0) rk_combine<W>(a, i, unew);
             store_cell<W, NV>(a.Unext, i, unew);
             if (nonphysical(a.gamma, unew) || bad) a.fault[inst] = 1;
 #pragma unroll
@@ -435,7 +432,7 @@ k_fused_step(
         } else if (bad) {
             a.fault[0] = 1;
         }
-        if constexpr (CFL) cfl_tile_lds<MULTI>(a, tile, i, dti, tmin);
+        if constexpr (CFL) cfl_tile_lds<MULTI>(a, tile, L.tb, L.tc, hasT ? i : a.Nint, dti, tmin);
     }
 }
 
@@ -447,14 +444,15 @@ __global__ __launch_bounds__(FUSED_TILE) void k_cfl_bound(PassArgs a) {
     int tfirst, step, tend;
     tile_walk(a, tfirst, step, tend);
     for (int tile = tfirst; tile < tend; tile += step) {
-        const int i = tile * FUSED_TILE + threadIdx.x;
+        const int tb = a.tile_start[tile], tc = a.tile_start[tile + 1] - tb;
+        const int i = tb + threadIdx.x;
         double dti = INFINITY;
-        if (i < a.Nint) {
+        if ((int)threadIdx.x < tc) {
             double u[NV];
             load_cell<W, NV>(a.U, i, u);
             dti = cfl_bound<T, MAXDEG>(a, i, u);
         }
-        cfl_tile<MULTI>(a, tile, i, dti, sh);
+        cfl_tile_range<MULTI>(a, tile, tb, tb + tc - 1, (int)threadIdx.x < tc ? i : a.Nint, dti, sh);
     }
 }
 

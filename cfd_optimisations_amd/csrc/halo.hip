@@ -46,8 +46,21 @@ __device__ __forceinline__ long long lower_bound_u64(const u64* a, long long n, 
     return lo;
 }
 
+// Tile of each interior cell (tiles are consecutive ranges tile_start[t]..).
+__global__ void k_cell_tiles(int Nint, int ntiles, const int* __restrict__ tile_start, int* __restrict__ ctile) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= Nint) return;
+    int lo = 0, hi = ntiles - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tile_start[mid] <= i) lo = mid; else hi = mid - 1;
+    }
+    ctile[i] = lo;
+}
+
 // H1 candidates: interior neighbours in another tile.
-__global__ void k_h1_keys(int Nint, long long P, int nslot, int TC, const int* __restrict__ cnb, u64* __restrict__ keys) {
+__global__ void k_h1_keys(int Nint, long long P, int nslot, const int* __restrict__ ctile, const int* __restrict__ cnb,
+                          u64* __restrict__ keys) {
     const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= (long long)nslot * Nint) return;
     const int k = (int)(q / Nint), i = (int)(q % Nint);
@@ -55,14 +68,14 @@ __global__ void k_h1_keys(int Nint, long long P, int nslot, int TC, const int* _
     u64 key = NO_KEY;
     if (x != SLOT_EMPTY) {
         const int j = x & SLOT_NB_MASK;
-        if (j < Nint && j / TC != i / TC) key = ((u64)(i / TC) << 32) | (u64)j;
+        if (j < Nint && ctile[j] != ctile[i]) key = ((u64)ctile[i] << 32) | (u64)j;
     }
     keys[q] = key;
 }
 
 // S2 candidates: ghosts of tile cells (class 0), and neighbours of H1 cells
 // that are neither in the tile nor in its H1 list (class 1).
-__global__ void k_s2_keys(int Nint, long long P, int nslot, int TC, const int* __restrict__ cnb,
+__global__ void k_s2_keys(int Nint, long long P, int nslot, const int* __restrict__ ctile, const int* __restrict__ cnb,
                           const u64* __restrict__ h1keys, long long nh1, u64* __restrict__ keys) {
     const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long n_own = (long long)nslot * Nint;
@@ -73,7 +86,7 @@ __global__ void k_s2_keys(int Nint, long long P, int nslot, int TC, const int* _
         const int x = cnb[k * P + i];
         if (x != SLOT_EMPTY) {
             const int j = x & SLOT_NB_MASK;
-            if (j >= Nint) key = ((u64)(i / TC) << 32) | (u64)j;
+            if (j >= Nint) key = ((u64)ctile[i] << 32) | (u64)j;
         }
     } else {
         const long long r = q - n_own;
@@ -86,7 +99,7 @@ __global__ void k_s2_keys(int Nint, long long P, int nslot, int TC, const int* _
         if (x != SLOT_EMPTY) {
             const int j = x & SLOT_NB_MASK;
             bool skip = false;
-            if (j < Nint && (u64)(j / TC) == t) skip = true;
+            if (j < Nint && (u64)ctile[j] == t) skip = true;
             if (!skip && j < Nint) {
                 const long long p = lower_bound_u64(h1keys, nh1, (t << 32) | (u64)j);
                 skip = p < nh1 && h1keys[p] == ((t << 32) | (u64)j);
@@ -143,60 +156,113 @@ long long sort_unique(u64*& keys, long long n, hipStream_t st) {
 
 }  // namespace
 
-void build_halo_tables(int Nint, int NcPad, int nslot, const int* cnb, int TC, HaloTables& out, hipStream_t st) {
-    free_halo_tables(out);
-    out.TC = TC;
-    out.ntiles = (Nint + TC - 1) / TC;
-    const long long P = NcPad;
+namespace {
+
+// Lists of one tiling (tile_start on the host); per-tile sizes returned.
+void build_lists(int Nint, long long P, int nslot, const int* cnb, const std::vector<int>& ts, HaloTables& out,
+                 std::vector<int>& nh, std::vector<int>& ns, std::vector<int>& ng, hipStream_t st) {
+    const int ntiles = (int)ts.size() - 1;
+    out.ntiles = ntiles;
+    out.tile_start = dalloc<int>(ntiles + 1);
+    HIPCHK(hipMemcpyAsync(out.tile_start, ts.data(), sizeof(int) * (ntiles + 1), hipMemcpyHostToDevice, st));
+    int* ctile = dalloc<int>(Nint);
+    k_cell_tiles<<<nblk(Nint), 256, 0, st>>>(Nint, ntiles, out.tile_start, ctile);
     const long long n1 = (long long)nslot * Nint;
     // ---- H1 ---------------------------------------------------------------------------
     u64* k1 = dalloc<u64>(n1);
-    k_h1_keys<<<nblk(n1), 256, 0, st>>>(Nint, P, nslot, TC, cnb, k1);
+    k_h1_keys<<<nblk(n1), 256, 0, st>>>(Nint, P, nslot, ctile, cnb, k1);
     HIPCHK(hipGetLastError());
     const long long nh1 = sort_unique(k1, n1, st);
     out.nh1 = nh1;
-    out.h1_off = dalloc<int>(out.ntiles + 1);
+    out.h1_off = dalloc<int>(ntiles + 1);
     out.h1 = dalloc<int>(nh1);
-    k_tile_bounds<<<nblk(out.ntiles + 1), 256, 0, st>>>(out.ntiles, k1, nh1, 0, out.h1_off);
+    k_tile_bounds<<<nblk(ntiles + 1), 256, 0, st>>>(ntiles, k1, nh1, 0, out.h1_off);
     if (nh1) k_key_ids<<<nblk(nh1), 256, 0, st>>>(nh1, k1, out.h1);
     // ---- S2 ---------------------------------------------------------------------------
     const long long n2 = n1 + (long long)nslot * nh1;
     u64* k2 = dalloc<u64>(n2);
-    k_s2_keys<<<nblk(n2), 256, 0, st>>>(Nint, P, nslot, TC, cnb, k1, nh1, k2);
+    k_s2_keys<<<nblk(n2), 256, 0, st>>>(Nint, P, nslot, ctile, cnb, k1, nh1, k2);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(st));
     (void)hipFree(k1);
+    (void)hipFree(ctile);
     const long long ns2 = sort_unique(k2, n2, st);
     out.ns2 = ns2;
-    out.s_off = dalloc<int>(out.ntiles + 1);
-    out.s_ngt = dalloc<int>(out.ntiles + 1);
+    out.s_off = dalloc<int>(ntiles + 1);
+    out.s_ngt = dalloc<int>(ntiles + 1);
     out.s2 = dalloc<int>(ns2);
-    k_tile_bounds<<<nblk(out.ntiles + 1), 256, 0, st>>>(out.ntiles, k2, ns2, 0, out.s_off);
-    k_tile_bounds<<<nblk(out.ntiles + 1), 256, 0, st>>>(out.ntiles, k2, ns2, CLS_REST, out.s_ngt);
+    k_tile_bounds<<<nblk(ntiles + 1), 256, 0, st>>>(ntiles, k2, ns2, 0, out.s_off);
+    k_tile_bounds<<<nblk(ntiles + 1), 256, 0, st>>>(ntiles, k2, ns2, CLS_REST, out.s_ngt);
     if (ns2) k_key_ids<<<nblk(ns2), 256, 0, st>>>(ns2, k2, out.s2);
     HIPCHK(hipGetLastError());
-    // ---- sizes (host) ------------------------------------------------------------------
-    std::vector<int> h1o(out.ntiles + 1), so(out.ntiles + 1), sg(out.ntiles + 1);
-    HIPCHK(hipMemcpyAsync(h1o.data(), out.h1_off, sizeof(int) * (out.ntiles + 1), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(so.data(), out.s_off, sizeof(int) * (out.ntiles + 1), hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(sg.data(), out.s_ngt, sizeof(int) * (out.ntiles + 1), hipMemcpyDeviceToHost, st));
+    std::vector<int> h1o(ntiles + 1), so(ntiles + 1), sg(ntiles + 1);
+    HIPCHK(hipMemcpyAsync(h1o.data(), out.h1_off, sizeof(int) * (ntiles + 1), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(so.data(), out.s_off, sizeof(int) * (ntiles + 1), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(sg.data(), out.s_ngt, sizeof(int) * (ntiles + 1), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     (void)hipFree(k2);
-    for (int t = 0; t < out.ntiles; t++) {
-        sg[t] -= so[t];  // ghosts of the tile's cells: [s_off, first class-1 entry)
-        const int tc = std::min(TC, Nint - t * TC);
-        const int nh = h1o[t + 1] - h1o[t], ns = so[t + 1] - so[t];
-        out.smax = std::max(out.smax, tc + nh + ns);
-        out.fsmax = std::max(out.fsmax, tc + nh + sg[t]);
-        out.h1max = std::max(out.h1max, nh);
-        out.s2max = std::max(out.s2max, ns);
+    nh.assign(ntiles, 0);
+    ns.assign(ntiles, 0);
+    ng.assign(ntiles, 0);
+    for (int t = 0; t < ntiles; t++) {
+        ng[t] = sg[t] - so[t];  // ghosts of the tile's cells: [s_off, first class-1 entry)
+        nh[t] = h1o[t + 1] - h1o[t];
+        ns[t] = so[t + 1] - so[t];
     }
-    HIPCHK(hipMemcpyAsync(out.s_ngt, sg.data(), sizeof(int) * out.ntiles, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(out.s_ngt, ng.data(), sizeof(int) * ntiles, hipMemcpyHostToDevice, st));
     HIPCHK(hipStreamSynchronize(st));
 }
 
+}  // namespace
+
+void build_halo_tables(int Nint, int NcPad, int nslot, const int* cnb, int TC, bool mixed, size_t lds_budget,
+                       HaloTables& out, hipStream_t st) {
+    free_halo_tables(out);
+    out.TC = TC;
+    std::vector<int> ts;
+    for (int c = 0; c < Nint; c += TC) ts.push_back(c);
+    ts.push_back(Nint);
+    std::vector<int> nh, ns, ng;
+    for (int round = 0;; round++) {
+        build_lists(Nint, NcPad, nslot, cnb, ts, out, nh, ns, ng, st);
+        out.smax = out.fsmax = out.h1max = out.s2max = 0;
+        const int ntiles = out.ntiles;
+        std::vector<size_t> bytes(ntiles);
+        for (int t = 0; t < ntiles; t++) {
+            const int tc = ts[t + 1] - ts[t];
+            out.smax = std::max(out.smax, tc + nh[t] + ns[t]);
+            out.fsmax = std::max(out.fsmax, tc + nh[t] + ng[t]);
+            out.h1max = std::max(out.h1max, nh[t]);
+            out.s2max = std::max(out.s2max, ns[t]);
+            bytes[t] = fused_lds_bytes(mixed, nslot, tc + nh[t] + ns[t], tc + nh[t] + ng[t], nh[t], ns[t]);
+        }
+        const size_t lds = fused_lds_bytes(mixed, nslot, out.smax, out.fsmax, out.h1max, out.s2max);
+        if (lds <= lds_budget || round == 5) break;
+        // halve the tiles whose own footprint is over budget (tightened a little
+        // each round, since the block's LDS takes the maximum of each list)
+        const size_t limit = lds_budget - lds_budget * (size_t)round / 20;
+        std::vector<int> nt;
+        nt.reserve(ts.size() + 64);
+        int split = 0;
+        for (int t = 0; t < ntiles; t++) {
+            nt.push_back(ts[t]);
+            const int tc = ts[t + 1] - ts[t];
+            if (bytes[t] > limit && tc >= 32) {
+                nt.push_back(ts[t] + tc / 2);
+                split++;
+            }
+        }
+        nt.push_back(Nint);
+        if (!split) break;
+        ts.swap(nt);
+        free_halo_tables(out);
+        out.TC = TC;
+    }
+    out.split_tiles = out.ntiles - (Nint + TC - 1) / TC;
+}
+
 void free_halo_tables(HaloTables& t) {
-    for (void* p : {(void*)t.h1_off, (void*)t.h1, (void*)t.s_off, (void*)t.s_ngt, (void*)t.s2})
+    for (void* p : {(void*)t.tile_start, (void*)t.h1_off, (void*)t.h1, (void*)t.s_off, (void*)t.s_ngt, (void*)t.s2})
         if (p) (void)hipFree(p);
     t = HaloTables{};
 }

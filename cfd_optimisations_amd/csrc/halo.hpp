@@ -19,6 +19,8 @@ namespace hcfd {
 
 struct HaloTables {
     int TC = 0, ntiles = 0;
+    int* tile_start = nullptr;  // [ntiles + 1] first cell of each tile (tiles hold at most TC cells)
+    int split_tiles = 0;        // tiles added by halving tiles whose halo exceeded the LDS budget
     int* h1_off = nullptr;  // [ntiles + 1]
     int* h1 = nullptr;      // H1 cell ids, tile by tile
     int* s_off = nullptr;   // [ntiles + 1]
@@ -31,8 +33,12 @@ struct HaloTables {
 };
 
 // Builds the tables from the neighbour planes cnb ([nslot][NcPad], neighbour |
-// twin << 29 or SLOT_EMPTY) of the Nint interior cells.
-void build_halo_tables(int Nint, int NcPad, int nslot, const int* cnb, int TC, HaloTables& out, hipStream_t st);
+// twin << 29 or SLOT_EMPTY) of the Nint interior cells.  Tiles start as runs of
+// TC consecutive cells; a tile whose staged cells would need more LDS than
+// lds_budget (the share of one block at the intended occupancy) is halved, and
+// the lists rebuilt, for a few rounds.
+void build_halo_tables(int Nint, int NcPad, int nslot, const int* cnb, int TC, bool mixed, size_t lds_budget,
+                       HaloTables& out, hipStream_t st);
 void free_halo_tables(HaloTables& t);
 
 // Bytes of dynamic LDS the single-kernel step needs (block of 256 lanes):

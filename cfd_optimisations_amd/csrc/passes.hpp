@@ -36,14 +36,21 @@ struct PassArgs {
     const int* ghost_bc = nullptr;
     int* fault = nullptr;               // [ninst]
     // single-kernel step (halo tiles, see halo.hpp)
+    const int* tile_start = nullptr;    // [ntiles+1] variable tiles (single-kernel step, CFL bound), else null
     const int* h1_off = nullptr;
     const int* h1 = nullptr;
     const int* s_off = nullptr;
     const int* s_ngt = nullptr;
     const int* s2 = nullptr;
     int smax = 0, fsmax = 0, h1max = 0, s2max = 0;
-    double* tacc = nullptr;             // [ninst] simulated time, advanced by dt (CFL)
-    double* lastdt = nullptr;           // [ninst]
+    double* tacc = nullptr;             // [ninst] simulated time
+    double* lastdt = nullptr;           // [ninst] dt of the latest step
+    int advance = 0;                    // first stage of a step: block 0 of pass B (or of the single
+                                        // kernel) adds dt to tacc and stores it in lastdt
+    // Multi-stage (SSP-RK) steps: a stage after the first writes
+    // rk_a * U0 + rk_b * (U + dt R), U0 = the step's starting state; null U0 = U + dt R.
+    const double* U0 = nullptr;
+    double rk_a = 0.0, rk_b = 1.0;
 };
 
 // Variant selectors.
@@ -58,10 +65,8 @@ struct PassVariant {
 // Launch the passes with `grid` x `block` threads on `st`.
 hipError_t launch_pass_a(const PassVariant& v, const PassArgs& a, int grid, int block, hipStream_t st);
 hipError_t launch_pass_b(const PassVariant& v, const PassArgs& a, int grid, int block, hipStream_t st);
-// Per-instance CFL reduction: dt[b] = cfl * min(...); with advance, also
-// t[b] += dt[b] and last[b] = dt[b] (the single-kernel step advances them itself).
-hipError_t launch_dt_reduce(const PassArgs& a, double cfl, double* dt, double* t, double* last, hipStream_t st,
-                            bool advance = true);
+// Per-instance CFL reduction: dt[b] = cfl * min over the instance's tile partials.
+hipError_t launch_dt_reduce(const PassArgs& a, double cfl, double* dt, hipStream_t st);
 // Single-kernel step over halo tiles of 256 cells, block 256, `lds` bytes of dynamic LDS.
 hipError_t launch_fused_step(const PassVariant& v, const PassArgs& a, int grid, size_t lds, hipStream_t st);
 // CFL bound of the current state (tile partials), block 256.

@@ -50,14 +50,38 @@ hipError_t launch_pass_b(const PassVariant& v, const PassArgs& a, int grid, int 
 }
 
 // Per-instance CFL reduction over the tile partials of pass A (block b <-> instance b).
-__global__ __launch_bounds__(1024) void k_dt_reduce(PassArgs a, double cfl, double* __restrict__ dt,
-                                                    double* __restrict__ t, double* __restrict__ last, int advance) {
+__global__ __launch_bounds__(1024) void k_dt_reduce(PassArgs a, double cfl, double* __restrict__ dt) {
     __shared__ double sh[1024 / WAVE];
     const int b = blockIdx.x;
     const int c0 = a.inst_off[b], c1 = a.inst_off[b + 1];
     double m = INFINITY;
     if (c1 > c0) {
-        const int t0 = c0 / a.tile_cells, t1 = (c1 - 1) / a.tile_cells;
+        // tiles of tile_cells cells, or the variable tiles of tile_start (one
+        // instance: every tile; several: thread 0 searches and shares the range)
+        __shared__ int s_t[2];
+        int t0, t1;
+        if (!a.tile_start) {
+            t0 = c0 / a.tile_cells;
+            t1 = (c1 - 1) / a.tile_cells;
+        } else if (a.ninst == 1) {
+            t0 = 0;
+            t1 = a.ntiles - 1;
+        } else {
+            if (threadIdx.x == 0) {
+                for (int e = 0; e < 2; e++) {
+                    const int c = e == 0 ? c0 : c1 - 1;
+                    int lo = 0, hi = a.ntiles - 1;
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (a.tile_start[mid] <= c) lo = mid; else hi = mid - 1;
+                    }
+                    s_t[e] = lo;
+                }
+            }
+            __syncthreads();
+            t0 = s_t[0];
+            t1 = s_t[1];
+        }
         if (a.ninst == 1) {
             // one instance: every partial is its tile's first; four loads in flight per lane
             double m1 = INFINITY, m2 = INFINITY, m3 = INFINITY;
@@ -73,8 +97,8 @@ __global__ __launch_bounds__(1024) void k_dt_reduce(PassArgs a, double cfl, doub
             m = fmin(fmin(m, m1), fmin(m2, m3));
         } else {
             for (int tile = t0 + (int)threadIdx.x; tile <= t1; tile += blockDim.x) {
-                const int f0 = tile * a.tile_cells;
-                const int l0 = min(f0 + a.tile_cells, a.Nint) - 1;
+                const int f0 = a.tile_start ? a.tile_start[tile] : tile * a.tile_cells;
+                const int l0 = a.tile_start ? a.tile_start[tile + 1] - 1 : min(f0 + a.tile_cells, a.Nint) - 1;
                 const int bf = instance_of(a.inst_off, a.ninst, f0);
                 const int bl = instance_of(a.inst_off, a.ninst, l0);
                 if (bf == b) m = fmin(m, a.pmin_first[tile]);
@@ -85,19 +109,13 @@ __global__ __launch_bounds__(1024) void k_dt_reduce(PassArgs a, double cfl, doub
     m = block_min(m, sh);
     if (threadIdx.x == 0) {
         m = fmin(m, __longlong_as_double((long long)a.dtacc[b]));
-        const double h = cfl * m;
-        dt[b] = h;
-        if (advance) {
-            t[b] += h;
-            last[b] = h;
-        }
+        dt[b] = cfl * m;
         a.dtacc[b] = dbits(INFINITY);
     }
 }
 
-hipError_t launch_dt_reduce(const PassArgs& a, double cfl, double* dt, double* t, double* last, hipStream_t st,
-                            bool advance) {
-    hipLaunchKernelGGL(k_dt_reduce, dim3(a.ninst), dim3(1024), 0, st, a, cfl, dt, t, last, advance ? 1 : 0);
+hipError_t launch_dt_reduce(const PassArgs& a, double cfl, double* dt, hipStream_t st) {
+    hipLaunchKernelGGL(k_dt_reduce, dim3(a.ninst), dim3(1024), 0, st, a, cfl, dt);
     return hipGetLastError();
 }
 

@@ -15,8 +15,9 @@
 //   pass B  per cell: Rusanov flux of each of its faces (Flux.cpp:52-109) from
 //           its own FS[k][cell] and the neighbour's FS[twin][nb] (twin = the
 //           face's slot in the neighbour's list), residual * 1/V, U += dt R
-//           (Euler.cpp:107-113), and the ghost states of its boundary faces
-//           from the new state (BoundaryConditions/{Symmetry,Wall,Inlet,Outlet}.cpp).
+//           (Euler.cpp:107-113) [+ the SSP-RK stage combination with the step's
+//           starting state], and the ghost states of its boundary faces from the
+//           new state (BoundaryConditions/{Symmetry,Wall,Inlet,Outlet}.cpp).
 //
 // Geometry is read from per-slot planes (cell-major, so each load of a wave is
 // one contiguous run) with the normal oriented out of the cell.  Each cell sums
@@ -72,16 +73,16 @@ __device__ __forceinline__ void tile_walk(const PassArgs& a, int& t0, int& step,
 
 __device__ __forceinline__ unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
 
-// CFL partials of one tile: the tile's first and last instance get block
-// minima; cells of any instance strictly inside the tile use atomics.
+// CFL partials of one tile of cells [c0, c1]: the tile's first and last
+// instance get block minima; cells of any instance strictly inside the tile use
+// atomics.  Lanes without a cell pass i >= Nint.
 template <bool MULTI>
-__device__ __forceinline__ void cfl_tile(const PassArgs& a, int tile, int i, double dti, double* sh) {
+__device__ __forceinline__ void cfl_tile_range(const PassArgs& a, int tile, int c0, int c1, int i, double dti,
+                                               double* sh) {
     if constexpr (!MULTI) {
         const double m = block_min(dti, sh);
         if (threadIdx.x == 0) a.pmin_first[tile] = m;
     } else {
-        const int c0 = tile * a.tile_cells;
-        const int c1 = min(c0 + a.tile_cells, a.Nint) - 1;
         const int bf = instance_of(a.inst_off, a.ninst, c0);
         const int bl = instance_of(a.inst_off, a.ninst, c1);
         double vf = INFINITY, vl = INFINITY;
@@ -98,6 +99,13 @@ __device__ __forceinline__ void cfl_tile(const PassArgs& a, int tile, int i, dou
             a.pmin_last[tile] = ml;
         }
     }
+}
+
+// Tiles of tile_cells consecutive cells (the two-pass kernels).
+template <bool MULTI>
+__device__ __forceinline__ void cfl_tile(const PassArgs& a, int tile, int i, double dti, double* sh) {
+    const int c0 = tile * a.tile_cells;
+    cfl_tile_range<MULTI>(a, tile, c0, min(c0 + a.tile_cells, a.Nint) - 1, i, dti, sh);
 }
 
 // Face-state planes: FP64 slot k of cell i is two double2 at planes 2k, 2k+1;
@@ -450,6 +458,26 @@ __device__ __forceinline__ void tile_or_global_fs(const T* fs, long long P, cons
     else load_fs(fs, P, k, j, v);
 }
 
+// The step's clocks, on the device in both dt modes (the same FP64 additions
+// as Euler::run's t += dt, src/Base/Euler.cpp:96): block 0 of the first stage.
+__device__ __forceinline__ void advance_clocks(const PassArgs& a) {
+    if (a.advance && blockIdx.x == 0)
+        for (int b = threadIdx.x; b < a.ninst; b += blockDim.x) {
+            a.tacc[b] += a.dt[b];
+            a.lastdt[b] = a.dt[b];
+        }
+}
+
+// Stage combination of a multi-stage step: unew = rk_a * U0 + rk_b * unew, the
+// two products rounded separately (no contraction), as the Shu-Osher form reads.
+template <int W>
+__device__ __forceinline__ void rk_combine(const PassArgs& a, int i, double unew[NV]) {
+    double u0[NV];
+    load_cell<W, NV>(a.U0, i, u0);
+#pragma unroll
+    for (int v = 0; v < NV; v++) unew[v] = a.rk_a * u0[v] + a.rk_b * unew[v];
+}
+
 template <int W, int MAXDEG>
 __device__ __forceinline__ void pass_b_cell_fp64(const PassArgs& a, int i, double dt, int inst,
                                                  const double2* __restrict__ sF, int tb, int tc) {
@@ -484,6 +512,7 @@ __device__ __forceinline__ void pass_b_cell_fp64(const PassArgs& a, int i, doubl
         R[v] = R[v] * iv;
         unew[v] = u[v] + dt * R[v];
     }
+    if (a.U0) rk_combine<W>(a, i, unew);
     store_cell<W, NV>(a.Unext, i, unew);
     if (nonphysical(a.gamma, unew)) a.fault[inst] = 1;
 #pragma unroll
@@ -544,6 +573,7 @@ __device__ __forceinline__ void pass_b_cell_mixed(const PassArgs& a, int i, doub
     double unew[NV];
 #pragma unroll
     for (int v = 0; v < NV; v++) unew[v] = u[v] + dt * (R[v].value() * (double)iv);
+    if (a.U0) rk_combine<W>(a, i, unew);
     store_cell<W, NV>(a.Unext, i, unew);
     if (nonphysical(a.gamma, unew)) a.fault[inst] = 1;
 #pragma unroll
@@ -567,6 +597,7 @@ __global__ __launch_bounds__(256) HCFD_WAVES_B_ATTR void k_pass_b(PassArgs a) {
     constexpr int PLANES = MIXED ? MAXDEG : 2 * MAXDEG;
     __shared__ S sF[PLANES * LDS_TILE];
     const long long P = a.NcPad;
+    advance_clocks(a);
     int tfirst, step, tend;
     tile_walk(a, tfirst, step, tend);
     for (int tile = tfirst; tile < tend; tile += step) {

@@ -228,12 +228,22 @@ void Euler::initialise(const double QIC[NVARS]) {
 }
 
 // Euler::run (src/Base/Euler.cpp:79-105): the same t += dt bookkeeping and
-// output cadence; the steps between outputs run as one fused device call.
+// output cadence; the steps between outputs are one device call.  An output
+// snapshot is taken on the device and copied to host memory on a second stream
+// while the next steps run; the file of output k is written by the host after
+// output k+1's steps are enqueued, so writing overlaps the device's work.
 void Euler::run(double T, double dt, double dt_out) {
+    hcfd_ctx* ctx = domain.ctx();
     double t = 0.0;
     double t_out = dt_out;
-    compute_prims();
-    writer->write_soln();
+    int slot = 0;
+    bool pending = false;  // slot ^ 1 holds a snapshot not yet written
+    auto write = [&](int s) {
+        check(hcfd_output_wait(ctx, s, Q, nullptr), "hcfd_output_wait");
+        writer->write_soln();
+    };
+    check(hcfd_output_begin(ctx, slot), "hcfd_output_begin");
+    write(slot);
     while (t < T) {
         int n = 0;
         bool out = false;
@@ -242,13 +252,17 @@ void Euler::run(double T, double dt, double dt_out) {
             n++;
             out = (t >= t_out);
         } while (!out && t < T);
-        check(hcfd_step(domain.ctx(), n, dt, 0.0, nullptr), "hcfd_step");
+        check(hcfd_step_async(ctx, n, dt, 0.0), "hcfd_step_async");
         if (out) {
-            compute_prims();
-            writer->write_soln();
+            slot ^= 1;
+            check(hcfd_output_begin(ctx, slot), "hcfd_output_begin");
+            if (pending) write(slot ^ 1);
+            pending = true;
             t_out += dt_out;
         }
     }
+    if (pending) write(slot);
+    check(hcfd_sync(ctx, nullptr), "hcfd_sync");
 }
 
 // Euler::compute_prims (src/Base/Euler.cpp:115-136), computed on the device.

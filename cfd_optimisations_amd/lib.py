@@ -104,6 +104,8 @@ def load() -> ctypes.CDLL:
         "hcfd_get_fault": [P, P],
         "hcfd_time_steps": [P, I, D, D, P],
         "hcfd_set_schedule": [P, P],
+        "hcfd_set_integrator": [P, I],
+        "hcfd_get_integrator": [P, P],
         "hcfd_get_schedule": [P, P],
         "hcfd_autotune": [P, I, D, ctypes.c_char_p, P],
         "hcfd_batch_get_state": [P, I, P],
@@ -118,6 +120,12 @@ def load() -> ctypes.CDLL:
         "hcfd_meshgen_riemann": [I, I, D, ctypes.c_ulonglong, I, P],
         "hcfd_meshgen_ramp": [I, I, D, D, D, D, D, ctypes.c_ulonglong, I, P],
         "hcfd_write_hsln": [ctypes.c_char_p, I, P],
+        "hcfd_read_hsln": [ctypes.c_char_p, P, P],
+        "hcfd_write_vtu": [ctypes.c_char_p, P, P],
+        "hcfd_step_async": [P, I, D, D],
+        "hcfd_sync": [P, P],
+        "hcfd_output_begin": [P, I],
+        "hcfd_output_wait": [P, I, P, P],
         "hcfd_mesh_import_msh": [ctypes.c_char_p, P],
         "hcfd_stage_bc": [P, I, I],
         "hcfd_set_gamma": [P, D],
@@ -404,6 +412,24 @@ class Context:
         check(load().hcfd_step(self.h, nsteps, dt, cfl, ctypes.byref(t)))
         return t.value
 
+    def step_async(self, nsteps: int, dt: float = 0.0, cfl: float = 0.0) -> None:
+        check(load().hcfd_step_async(self.h, nsteps, dt, cfl))
+
+    def sync(self) -> float:
+        t = D(0.0)
+        check(load().hcfd_sync(self.h, ctypes.byref(t)))
+        return t.value
+
+    def output_begin(self, slot: int) -> None:
+        check(load().hcfd_output_begin(self.h, slot))
+
+    def output_wait(self, slot: int):
+        """(primitives (4, Nc) in caller order, simulated time) of the slot's snapshot."""
+        Q = np.zeros((4, self.Nc))
+        t = D(0.0)
+        check(load().hcfd_output_wait(self.h, slot, _rows(Q), ctypes.byref(t)))
+        return Q, t.value
+
     def last_dt(self) -> float:
         d = D(0.0)
         check(load().hcfd_get_last_dt(self.h, ctypes.byref(d)))
@@ -418,6 +444,15 @@ class Context:
         ms = (ctypes.c_float * 4)()
         check(load().hcfd_time_steps(self.h, nsteps, dt, cfl, ms))
         return {"wall_ms": ms[0], "pass_a_ms": ms[1], "pass_b_ms": ms[2], "dt_reduce_ms": ms[3]}
+
+    def set_integrator(self, stages: int) -> None:
+        """1 = forward Euler (the reference's), 2 = SSP-RK2, 3 = SSP-RK3."""
+        check(load().hcfd_set_integrator(self.h, stages))
+
+    def integrator(self) -> int:
+        n = I(0)
+        check(load().hcfd_get_integrator(self.h, ctypes.byref(n)))
+        return n.value
 
     def schedule(self) -> dict:
         s = Schedule()
@@ -462,3 +497,20 @@ class Context:
 def write_hsln(path: str, Q: np.ndarray) -> None:
     Q = np.ascontiguousarray(Q, dtype=np.float64)
     check(load().hcfd_write_hsln(path.encode(), Q.shape[1], _rows(Q)))
+
+
+def read_hsln(path: str) -> np.ndarray:
+    """Primitives (4, Nc) of a .hsln file, through the library's reader."""
+    n = I(0)
+    check(load().hcfd_read_hsln(path.encode(), ctypes.byref(n), None))
+    Q = np.zeros((4, n.value))
+    check(load().hcfd_read_hsln(path.encode(), ctypes.byref(n), _rows(Q)))
+    return Q
+
+
+def write_vtu(path: str, mesh, Q: np.ndarray) -> None:
+    """VTU file of a mesh (NativeMesh, HMesh or MeshView) with primitives Q (4, Nc)."""
+    v, keep = as_view(mesh)
+    Q = np.ascontiguousarray(Q, dtype=np.float64)
+    check(load().hcfd_write_vtu(path.encode(), ctypes.byref(v), _rows(Q)))
+    del keep

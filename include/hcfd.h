@@ -110,9 +110,11 @@ typedef struct hcfd_info {
     int ncolours;        /* face colours of the conflict-free colouring */
     long long device_bytes;
     hcfd_schedule sched;
-    int fused_available;  /* the halo tiles of this mesh fit in LDS (see hcfd_schedule.fused) */
+    int fused_available;  /* the halo tiles of this mesh fit in LDS (see hcfd_schedule.fused); the
+                             single-kernel step is the default FP64 schedule when, in addition,
+                             at most 5% of the 256-cell tiles had to be split to fit */
     int fused_lds_bytes;  /* LDS per block of the single-kernel step */
-    int halo_max_cells;   /* most cells one tile stages: its 256 + halo ring + their neighbours */
+    int halo_max_cells;   /* most cells one tile stages: its own (at most 256) + halo ring + their neighbours */
 } hcfd_info;
 
 typedef struct hcfd_ctx hcfd_ctx;
@@ -193,21 +195,49 @@ int hcfd_set_gamma(hcfd_ctx* ctx, double gamma);
 /* ---- fused stepping (the hot path) --------------------------------------- */
 
 /* nsteps steps of Euler::run's loop body (src/Base/Euler.cpp:85-97) as two
- * fused passes per step.  cfl > 0: dt = cfl * min_i 1/(iv_i sum_f A_f(|u.n|+c))
+ * fused passes (or one single-kernel launch) per step and stage.  cfl > 0: dt = cfl * min_i 1/(iv_i sum_f A_f(|u.n|+c))
  * recomputed every step (no reference counterpart); otherwise fixed dt.
  * t_inout (may be NULL) is advanced by the sum of the steps' dt. */
 int hcfd_step(hcfd_ctx* ctx, int nsteps, double dt, double cfl, double* t_inout);
+/* hcfd_step without the wait: enqueues the steps on the context's stream and
+ * returns; hcfd_sync waits for them and gives the simulated time (t may be
+ * NULL).  Lets the host write output while the device steps on. */
+int hcfd_step_async(hcfd_ctx* ctx, int nsteps, double dt, double cfl);
+int hcfd_sync(hcfd_ctx* ctx, double* t);
+/* Output while stepping (the reference computes primitives and writes them
+ * with the solver stopped, Euler::run, src/Base/Euler.cpp:99-103):
+ * hcfd_output_begin enqueues Euler::compute_prims (src/Base/Euler.cpp:115-136)
+ * of the current state into device slot `slot` (0 or 1), then, on a second
+ * stream, its copy to pinned host memory, and returns at once; steps enqueued
+ * afterwards overlap the copy.  hcfd_output_wait blocks until the slot's copy
+ * has landed and gives the primitives (caller order, all cells) and the
+ * simulated time of the snapshot (Q or t may be NULL).  Each begin needs its
+ * wait before the slot is begun again. */
+int hcfd_output_begin(hcfd_ctx* ctx, int slot);
+int hcfd_output_wait(hcfd_ctx* ctx, int slot, double* const Q[4], double* t);
 /* dt of the most recent step. */
 int hcfd_get_last_dt(hcfd_ctx* ctx, double* dt);
 /* 1 if any step since the last call produced rho <= 0, p <= 0 or NaN. */
 int hcfd_get_fault(hcfd_ctx* ctx, int* fault);
+
+/* Time integrator of hcfd_step / hcfd_time_steps: stages = 1 forward Euler
+ * (Euler::run, the reference's only integrator, includes/Base/TimeIntegrators.hpp:11-45),
+ * 2 SSP-RK2, 3 SSP-RK3 in Shu-Osher form (no reference counterpart):
+ *   RK2: U1 = E(U^n); U^{n+1} = 1/2 U^n + 1/2 E(U1)
+ *   RK3: U1 = E(U^n); U2 = 3/4 U^n + 1/4 E(U1); U^{n+1} = 1/3 U^n + 2/3 E(U2)
+ * with E(W) = W + dt R(W) the forward-Euler stage (ghost cells set from each
+ * stage's interior) and dt the fixed dt or the CFL step of U^n.  Each stage is
+ * one more pass over the mesh; choosing 2 or 3 allocates a third state buffer. */
+int hcfd_set_integrator(hcfd_ctx* ctx, int stages);
+int hcfd_get_integrator(hcfd_ctx* ctx, int* stages);
 
 /* Runs nsteps steps on the context's stream with a HIP event pair around
  * every launch.  ms_out[0] = first to last event (wall of the nsteps steps);
  * ms_out[1] = mean pass A (gradient + limiter (+ CFL bound)), or mean of the
  * single-kernel step when the schedule is fused; ms_out[2] = mean pass B
  * (flux + update + ghost BCs), 0 when fused; ms_out[3] = mean CFL reduction
- * (0 for fixed dt). */
+ * (0 for fixed dt).  With a multi-stage integrator ms_out[1] and ms_out[3] are
+ * those of the first stage and ms_out[2] the rest of the step. */
 int hcfd_time_steps(hcfd_ctx* ctx, int nsteps, double dt, double cfl, float* ms_out);
 
 /* Device copy ceiling: a plain 16-byte-per-lane copy kernel over `bytes` bytes
@@ -264,6 +294,15 @@ int hcfd_meshgen_ramp(int nx, int ny, double length, double height, double x_ram
 /* Writer::write_soln (src/Writer.cpp:22-39): <fname>_<runcount>.hsln with the
  * primitive variables of all Nc cells. */
 int hcfd_write_hsln(const char* path, int Nc, const double* const Q[4]);
+/* The .hsln reader of interface/Export.py:14-32 (read_soln), for restarts:
+ * checks nvars == 4 and the file size, sets *Nc and, when Q is non-NULL, fills
+ * Q[v][0..Nc).  A restart is hcfd_initialise_prims with the arrays read. */
+int hcfd_read_hsln(const char* path, int* Nc, double* const Q[4]);
+/* VTU export without the vtk package, the output of interface/Export.py:34-81
+ * (write_vtk): all cells (ghost line cells included) with Density, Pressure
+ * and Velocity (3 components) as cell data; XML UnstructuredGrid with raw
+ * appended binary blocks. */
+int hcfd_write_vtu(const char* path, const hcfd_mesh_view* mesh, const double* const Q[4]);
 
 /* gmsh v4 ASCII .msh -> mesh with the reference importer's conventions
  * (interface/Mesh.py:21-127, 159-378): ghost line cells on the right of BC

@@ -22,6 +22,7 @@
  *
  * Extensions that have no reference counterpart (marked NEW below):
  *   - orc_cfl_dt: the CFL time-step formula used by the GPU path;
+ *   - orc_run_ssprk: SSP-RK2/RK3 steps built from the forward-Euler stage;
  *   - SubOutlet follows the corrected form of the reference's CAoS layout
  *     (src/CAoS/BoundaryConditions/Outlet.cpp:15-26) because the Base form
  *     reads rul[-1] (src/Base/BoundaryConditions/Outlet.cpp:26-28).
@@ -35,6 +36,9 @@
  *     consistency F(U,U)=E(U).n, limiter bounds, conservation).
  *   orc_cfl_dt (NEW): closed form on a uniform right-triangle mesh with a
  *     uniform state, and brute force in Python on tiny meshes.
+ *   orc_run_ssprk (NEW): each step equals the textbook combinations of the
+ *     reference's own forward-Euler steps (oracle/_ref), bit for bit; its
+ *     temporal order (errors falling as h^2 and h^3) on a smooth flow.
  *   SubInlet/SubOutlet/Wall: closed-form ghost states in the tests (the
  *     golden case only exercises Symmetry, SupInlet and SupOutlet).
  */
@@ -506,6 +510,53 @@ double orc_run(orc_dom* d, const int* bctype, const double* qbc, double gma, dou
         t += h;
     }
     free(G); free(phi); free(R);
+    return t;
+}
+
+/* NEW: strong-stability-preserving Runge-Kutta steps (Shu and Osher, J. Comput.
+ * Phys. 77 (1988) 439-471; Gottlieb, Shu and Tadmor, SIAM Rev. 43 (2001) 89-112)
+ * built from the forward-Euler stage above: with E(W) = W + h R(W) (the update of
+ * orc_run, ghost cells then set from the interior),
+ *   stages 1: U^{n+1} = E(U^n)                                  (forward Euler)
+ *   stages 2: U1 = E(U^n);  U^{n+1} = 1/2 U^n + 1/2 E(U1)             (SSP-RK2)
+ *   stages 3: U1 = E(U^n);  U2 = 3/4 U^n + 1/4 E(U1);
+ *             U^{n+1} = 1/3 U^n + 2/3 E(U2)                           (SSP-RK3)
+ * each combination evaluated as a*U^n + b*E with the two products rounded
+ * separately.  h is the fixed dt, or the CFL step of U^n when cfl > 0; the
+ * reference has forward Euler only (includes/Base/TimeIntegrators.hpp:11-45).
+ * Returns the simulated time, or -1 for an unsupported stage count. */
+double orc_run_ssprk(orc_dom* d, const int* bctype, const double* qbc, double gma, double* U, int nsteps,
+                     double dt, double cfl, int stages) {
+    static const double ca[4][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0.5, 0}, {0, 0.75, 1.0 / 3.0}};
+    static const double cb[4][3] = {{0, 0, 0}, {1, 0, 0}, {1, 0.5, 0}, {1, 0.25, 2.0 / 3.0}};
+    if (stages < 1 || stages > 3) return -1.0;
+    int Nc = d->Nc;
+    size_t n = (size_t)NV * Nc;
+    if (!d->bc_iv_zeroed) orc_zero_bc_iv(d);
+    double* G = (double*)calloc((size_t)NV * ND * Nc, sizeof(double));
+    double* phi = (double*)calloc(n, sizeof(double));
+    double* R = (double*)calloc(n, sizeof(double));
+    double* U0 = (double*)malloc(sizeof(double) * n);
+    double t = 0.0;
+    for (int s = 0; s < nsteps; s++) {
+        double h = (cfl > 0) ? orc_cfl_dt(d, gma, cfl, U) : dt;
+        memcpy(U0, U, sizeof(double) * n);
+        for (int k = 0; k < stages; k++) {
+            orc_gradient(d, U, G);
+            orc_limiter(d, U, G, phi);
+            orc_bc_grads(d, G);
+            orc_bc_limiter(d, phi);
+            orc_flux(d, gma, U, G, phi, R);
+            for (size_t q = 0; q < n; q++) {
+                double e = U[q] + h * R[q];
+                U[q] = (k == 0) ? e : ca[stages][k] * U0[q] + cb[stages][k] * e;
+            }
+            memset(R, 0, sizeof(double) * n);
+            orc_bc_cells(d, bctype, qbc, gma, U);
+        }
+        t += h;
+    }
+    free(G); free(phi); free(R); free(U0);
     return t;
 }
 

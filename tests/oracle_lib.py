@@ -61,6 +61,8 @@ def oracle_lib():
         lib.orc_cfl_dt.argtypes = [P, D, D, P]
         lib.orc_run.restype = D
         lib.orc_run.argtypes = [P, P, P, D, P, I, D, D]
+        lib.orc_run_ssprk.restype = D
+        lib.orc_run_ssprk.argtypes = [P, P, P, D, P, I, D, D, I]
         lib.orc_rusanov.argtypes = [D, P, P, P, P]
         lib.orc_euler_flux.argtypes = [D, P, P, P]
         lib.orc_cons_of_prims.argtypes = [D, P, P]
@@ -164,6 +166,15 @@ class OracleDomain:
     def run(self, U, nsteps, dt=0.0, cfl=0.0):
         U = np.array(U, copy=True)
         t = self.lib.orc_run(self.h, _ptr(self.bctypes), _ptr(self.qbc), self.gamma, _ptr(U), nsteps, dt, cfl)
+        return U, t
+
+    def run_ssprk(self, U, nsteps, stages, dt=0.0, cfl=0.0):
+        """nsteps SSP-RK steps of 1, 2 or 3 stages (1 = forward Euler)."""
+        U = np.array(U, copy=True)
+        t = self.lib.orc_run_ssprk(self.h, _ptr(self.bctypes), _ptr(self.qbc), self.gamma, _ptr(U), nsteps, dt,
+                                   cfl, stages)
+        if t < 0:
+            raise ValueError("unsupported stage count %d" % stages)
         return U, t
 
 

@@ -134,19 +134,25 @@ def test_wedge_reference_case_schedules(wedge_mesh, sched):
 
 def test_fixed_dt_after_cfl_steps():
     """Fixed dt, then CFL steps, then the same fixed dt again: the CFL reduction
-    overwrites the device dt, which must be reloaded."""
+    overwrites the device dt, which must be reloaded; the clock runs on through
+    both modes (t += dt per step, as Euler::run adds it) and the last dt follows."""
     case = C.riemann_case(30, 20, seed=4)
-    od, U0 = oracle_for(case)
-    U1, _ = od.run(U0, 5, dt=2e-4)
-    U2, _ = od.run(U1, 5, cfl=0.5)
-    U3, _ = od.run(U2, 5, dt=2e-4)
+    od, U = oracle_for(case)
+    t, hs = 0.0, []
+    for mode in ("dt", "cfl", "dt"):
+        for _ in range(5):
+            h = 2e-4 if mode == "dt" else od.cfl_dt(U, 0.5)
+            U = od.run(U, 1, dt=h)[0]
+            t += h
+            hs.append(h)
     for sched in (TWO_PASS, FUSED):
         with C.make_context(case, L.FP64, L.FLAG_ORDER_SFC) as ctx:
             ctx.set_schedule(**sched)
-            ctx.step(5, dt=2e-4)
+            assert ctx.step(5, dt=2e-4) == sum(hs[:5]) and ctx.last_dt() == 2e-4
             ctx.step(5, cfl=0.5)
-            ctx.step(5, dt=2e-4)
-            assert np.array_equal(ctx.get_state(), U3), sched
+            assert ctx.last_dt() == hs[9]
+            assert ctx.step(5, dt=2e-4) == t
+            assert np.array_equal(ctx.get_state(), U), sched
 
 
 @pytest.mark.parametrize("shuffle", [True, False])

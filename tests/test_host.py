@@ -218,3 +218,77 @@ def test_msh_import_errors(tmp_path):
     p.write_text("$MeshFormat\n2.2 0 8\n$EndMeshFormat\n")
     with pytest.raises(L.HcfdError):
         L.NativeMesh.import_msh(str(p))
+
+
+# ---------------------------------------------------------------------------
+# output path: .hsln reader (restarts) and VTU export
+# ---------------------------------------------------------------------------
+def test_hsln_reader_matches_golden_and_round_trips(tmp_path):
+    """The library's reader returns the shipped solutions exactly; write then
+    read is the identity; a truncated or foreign file is an I/O error."""
+    for name in ("Base_0.hsln", "Base_1.hsln", "VEC_1.hsln"):
+        path = os.path.join(GOLDEN, name)
+        names, Q = read_hsln(path)
+        assert np.array_equal(L.read_hsln(path), Q)
+    Q = np.random.default_rng(3).standard_normal((4, 777))
+    p = str(tmp_path / "x_0.hsln")
+    L.write_hsln(p, Q)
+    assert np.array_equal(L.read_hsln(p), Q)
+    raw = open(p, "rb").read()
+    open(tmp_path / "cut.hsln", "wb").write(raw[:-8])
+    with pytest.raises(L.HcfdError, match="size"):
+        L.read_hsln(str(tmp_path / "cut.hsln"))
+    bad = bytearray(raw)
+    bad[4:8] = np.int32(5).tobytes()   # nvars = 5
+    open(tmp_path / "bad.hsln", "wb").write(bytes(bad))
+    with pytest.raises(L.HcfdError, match="nvars"):
+        L.read_hsln(str(tmp_path / "bad.hsln"))
+
+
+def read_vtu(path):
+    """Minimal reader of the raw-appended VTU the library writes: the XML part is
+    parsed with ElementTree, each array's block is (UInt64 byte count, bytes)."""
+    import xml.etree.ElementTree as ET
+    raw = open(path, "rb").read()
+    mark = b'<AppendedData encoding="raw">'
+    head = raw.index(mark)
+    base = raw.index(b"_", head) + 1
+    root = ET.fromstring(raw[:head] + b"</VTKFile>")
+    assert root.get("type") == "UnstructuredGrid" and root.get("header_type") == "UInt64"
+    dt = {"Float64": np.float64, "Int32": np.int32, "UInt8": np.uint8}
+    out = {}
+    piece = root.find("UnstructuredGrid/Piece")
+    for da in piece.iter("DataArray"):
+        o = base + int(da.get("offset"))
+        n = int(np.frombuffer(raw[o:o + 8], np.uint64)[0])
+        a = np.frombuffer(raw[o + 8:o + 8 + n], dt[da.get("type")])
+        k = int(da.get("NumberOfComponents", "1"))
+        out[da.get("Name")] = a.reshape(-1, k) if k > 1 else a
+    return int(piece.get("NumberOfPoints")), int(piece.get("NumberOfCells")), out
+
+
+def test_vtu_export_matches_mesh_and_solution(tmp_path, wedge_mesh):
+    """interface/Export.py's write_vtk content: every cell with its type (line
+    ghosts, quads), points with z = 0, Density/Pressure and a 3-D Velocity."""
+    _, Q = read_hsln(os.path.join(GOLDEN, "Base_1.hsln"))
+    p = str(tmp_path / "w.vtu")
+    L.write_vtu(p, wedge_mesh, Q)
+    nn, nc, a = read_vtu(p)
+    m = wedge_mesh
+    assert (nn, nc) == (m.Nn, m.Nc)
+    assert np.array_equal(a["Points"][:, :2], m.nodes) and not a["Points"][:, 2].any()
+    assert np.array_equal(a["connectivity"], m.cn_con)
+    assert np.array_equal(a["offsets"], m.cn_off[1:])
+    nodes_per_cell = np.diff(m.cn_off)
+    assert np.array_equal(a["types"], np.select([nodes_per_cell == 2, nodes_per_cell == 3], [3, 5], 9))
+    assert (a["types"] == 3).sum() == 240 and (a["types"] == 9).sum() == 3500   # wedge: 240 ghosts, 3500 quads
+    assert np.array_equal(a["Density"], Q[0]) and np.array_equal(a["Pressure"], Q[3])
+    assert np.array_equal(a["Velocity"][:, :2], Q[1:3].T) and not a["Velocity"][:, 2].any()
+    # triangles from a generated mesh
+    nm = L.NativeMesh.riemann(5, 3, 0.1, 1, True)
+    hm = nm.to_hmesh()
+    Q2 = np.random.default_rng(1).random((4, hm.Nc))
+    L.write_vtu(str(tmp_path / "r.vtu"), nm, Q2)
+    _, _, b = read_vtu(str(tmp_path / "r.vtu"))
+    assert (b["types"] == 5).sum() == 30 and (b["types"] == 3).sum() == hm.Nc - 30
+    assert np.array_equal(b["Density"], Q2[0])

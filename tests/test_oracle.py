@@ -352,3 +352,87 @@ def test_step_count_semantics():
     lib = ol.oracle_lib()
     assert lib.orc_count_steps(1.0, 1e-3) == 1000
     assert lib.orc_count_steps(0.0, 1e-3) == 0
+
+
+# ---------------------------------------------------------------------------
+# SSP-RK steps (no reference counterpart: the reference integrates with forward
+# Euler only, includes/Base/TimeIntegrators.hpp:11-45)
+# ---------------------------------------------------------------------------
+def _ssprk_from_euler(E, bc, U, stages):
+    """One step of the Shu-Osher SSP-RK2/RK3 schemes written out from forward-Euler
+    steps E (Gottlieb, Shu and Tadmor, SIAM Rev. 43 (2001), eqs. 3.1 and 3.2)."""
+    if stages == 1:
+        return E(U)
+    if stages == 2:
+        return bc(0.5 * U + 0.5 * E(E(U)))
+    U2 = bc(0.75 * U + 0.25 * E(E(U)))
+    return bc((1.0 / 3.0) * U + (2.0 / 3.0) * E(U2))
+
+
+@need_ref
+@pytest.mark.parametrize("stages", [1, 2, 3])
+@pytest.mark.parametrize("mesh", ["wedge", "ramp"])
+def test_ssprk_equals_reference_euler_combinations(tmp_path, wedge, wedge_path, mesh, stages):
+    """Every SSP-RK step of the oracle equals the textbook combination of the
+    reference's own forward-Euler steps (sources built under oracle/_ref), bit for bit."""
+    from cfd_optimisations_amd import lib as L
+    if mesh == "wedge":
+        m, _, qic, qbc, _, bymap = wedge
+        path, dt = wedge_path, 1e-3
+    else:
+        nm = L.NativeMesh.ramp(24, 12, 1.0, 0.5, 0.2, 12.0, 0.2, 7, True)
+        path = str(tmp_path / "r.hmsh")
+        nm.save(path)
+        m = read_hmsh(path)
+        qic = np.array([1.0, 2.2, 0.1, 1.0 / G])
+        qbc = np.tile(qic, (4, 1))
+        qbc[3] = [1.1, 2.0, 0.0, 0.8]
+        bymap, dt = [0, 1, 4, 1, 3], 2e-4
+    bct = np.array([bymap[m.coltag[m.Ncol - m.NBCcol + b]] for b in range(m.NBCcol)])
+    od = ol.OracleDomain(m, G, bct, qbc)
+    rc = ol.RefCase(path, G, bymap, qbc, ol.REF_NOFMA_SO)
+    od.zero_bc_iv()
+    rc.zero_bc_iv()
+    U = od.run(od.initialise(qic), 60, dt=dt)[0]  # a developed flow: shocks, active limiter
+    W = U
+    for _ in range(3):
+        W = _ssprk_from_euler(lambda X: rc.run(X, 1, dt), rc.bc_cells, W, stages)
+    V, t = od.run_ssprk(U, 3, stages, dt=dt)
+    assert np.array_equal(V, W)
+    assert t == dt + dt + dt
+    # CFL mode: the step of U^n is used by every stage
+    h = od.cfl_dt(U, 0.4)
+    W = _ssprk_from_euler(lambda X: rc.run(X, 1, h), rc.bc_cells, U, stages)
+    V, t = od.run_ssprk(U, 1, stages, cfl=0.4)
+    assert np.array_equal(V, W) and t == h
+
+
+def test_ssprk_temporal_order():
+    """Local errors of one step fall as h^(s+1) for s = 1, 2, 3 stages.  The flow is
+    a small smooth perturbation of a free stream, scaled so that every limiter
+    increment stays below the limiter's 1e-9 threshold (phi = 1 everywhere) and
+    the semi-discrete right-hand side is smooth; the exact solution is replaced by
+    64 SSP-RK3 substeps."""
+    from cfd_optimisations_amd import lib as L
+    m = L.NativeMesh.riemann(12, 12, 0.2, 0, True).to_hmesh()
+    s, dl = 1e-4, 1e-5
+    q = np.array([s, 1.2, 0.4, s / G])
+    od = ol.OracleDomain(m, G, np.full(4, 4), np.tile(q, (4, 1)))
+    od.zero_bc_iv()
+    xc = od.geometry()["xc"]
+    Q = np.tile(q[:, None], (1, m.Nc))
+    Q[0] = s * (1.0 + dl * np.exp(-((xc[:, 0] - 0.4) ** 2 + (xc[:, 1] - 0.45) ** 2) / 0.05))
+    Q[3] = s * (1.0 + dl * np.sin(3 * xc[:, 0] + 2 * xc[:, 1])) / G
+    U0 = od.bc_cells(od.initialise_prims(Q))
+    scale = np.abs(U0).max(axis=1, keepdims=True) * dl
+    hs = [4e-3 / 2 ** k for k in range(4)]
+    for stages in (1, 2, 3):
+        errs = []
+        for h in hs:
+            ref = od.run_ssprk(U0, 64, 3, dt=h / 64)[0]
+            errs.append((np.abs(od.run_ssprk(U0, 1, stages, dt=h)[0] - ref) / scale).max())
+        rates = [math.log2(errs[k] / errs[k + 1]) for k in range(len(hs) - 1)]
+        assert min(rates) > stages + 0.8, (stages, rates)
+        assert max(rates) < stages + 1.2, (stages, rates)
+    with pytest.raises(ValueError):
+        od.run_ssprk(U0, 1, 4, dt=1e-3)
