@@ -265,15 +265,25 @@ def run_batch(name: str, steps: int, warmup: int, tune: bool, check: int, compar
     log(f"{name}: generated {len(cases)} meshes in {gen_s:.1f} s")
     flags = L.FLAG_GRAPH | L.FLAG_ORDER_SFC
 
-    def one(precision):
+    def one(precision, perturb=0.0, sched=None):
         t1 = time.perf_counter()
         ctx = C.make_batch(cases, precision, flags)
         setup = time.perf_counter() - t1
         inf = ctx.info()
         log(f"{'mixed' if precision else 'fp64'} batch: {inf.Nint:,} interior cells, built in {setup:.1f} s")
+        if perturb:
+            # every conserved value of every cell times (1 + perturb * N(0,1)), seeded
+            rng = np.random.default_rng(12345)
+            for k in range(ctx.ninst):
+                U = ctx.inst_state(k)
+                ctx.set_inst_state(k, U * (1.0 + perturb * rng.standard_normal(U.shape)))
+            log(f"initial states perturbed by {perturb:g} (relative)")
         order = np.argsort(ctx.inst_sizes)
         U0 = {int(k): ctx.inst_state(int(k)) for k in order[:check]}
-        sched = ctx.autotune(steps=4, cfl=cfl) if tune else ctx.schedule()
+        if sched is not None:
+            ctx.set_schedule(**sched)
+        else:
+            sched = ctx.autotune(steps=4, cfl=cfl) if tune else ctx.schedule()
         log(f"schedule {sched}")
         ctx.step(warmup, cfl=cfl)
         t1 = time.perf_counter()
@@ -341,11 +351,19 @@ def run_batch(name: str, steps: int, warmup: int, tune: bool, check: int, compar
         ctx.close()
         ctx64, rec64, _ = one(L.FP64)
         e = []
+        Ua = []
         for k in range(ctx64.ninst):
-            Ua = ctx64.inst_state(k)
-            e.append(float(np.linalg.norm(Um[k][0] - Ua[0]) / np.linalg.norm(Ua[0])))
+            Ua.append(ctx64.inst_state(k)[0])
+            e.append(float(np.linalg.norm(Um[k][0] - Ua[k]) / np.linalg.norm(Ua[k])))
         ctx64.close()
+        del Um
         e = np.array(e)
+        # the FP64 path's own sensitivity: the same run from initial states
+        # perturbed by 1e-7 (relative), i.e. about FP32 rounding once, at t = 0
+        ctxp, _, _ = one(L.FP64, perturb=1e-7, sched=rec64["schedule"])
+        fl = np.array([float(np.linalg.norm(ctxp.inst_state(k)[0] - Ua[k]) / np.linalg.norm(Ua[k]))
+                       for k in range(ctxp.ninst)])
+        ctxp.close()
         line["fp64_batch"] = {"value": rec64["value"], "ms_per_step": rec64["ms_per_step"],
                               "faulted_instances": rec64["faulted_instances"]}
         line["speedup_vs_fp64_batch"] = rec["value"] / rec64["value"]
@@ -354,6 +372,20 @@ def run_batch(name: str, steps: int, warmup: int, tune: bool, check: int, compar
                                       "median": float(np.median(e[fin])) if fin.any() else None,
                                       "non_finite_instances": int((~fin).sum()),
                                       "per_instance": [float(x) for x in e]}
+        ok = fin & np.isfinite(fl)
+        line["fp64_sensitivity"] = {
+            "what": "rho rel L2 of the FP64 batch run from initial states perturbed by 1e-7 (relative, seeded) "
+                    "against the unperturbed FP64 run, same steps",
+            "median": float(np.median(fl[ok])) if ok.any() else None,
+            "max": float(fl[ok].max()) if ok.any() else None,
+            "per_instance": [float(x) for x in fl]}
+        line["rho_error_bound"] = {
+            "bound": "rho rel L2 <= 1e-5",
+            "instances_within": int((e[fin] <= 1e-5).sum()), "finite_instances": int(fin.sum()),
+            "within_among_fp64_insensitive": int((e[ok & (fl <= 1e-6)] <= 1e-5).sum()),
+            "fp64_insensitive_instances": int((ok & (fl <= 1e-6)).sum()),
+            "median_ratio_to_fp64_sensitivity": float(np.median(e[ok] / np.maximum(fl[ok], 1e-300)))
+            if ok.any() else None}
     else:
         ctx.close()
     return line

@@ -128,6 +128,11 @@ struct hcfd_ctx {
     ~hcfd_ctx();
 };
 
+// Blocks per CU the FP32-mixed single-kernel step's LDS budget is sized for.
+#ifndef HCFD_FUSED_MIXED_BLOCKS
+#define HCFD_FUSED_MIXED_BLOCKS 4
+#endif
+
 namespace {
 
 void free_ptr(void* p) {
@@ -316,7 +321,7 @@ hcfd_ctx* create_ctx(int n, const hcfd_mesh_view* views, int precision, int flag
     // LDS share of one block at the occupancy the registers allow: FP64 three
     // blocks of four wavefronts per CU (two for octagon meshes), FP32-mixed four
     {
-        const int blocks = precision == HCFD_MIXED ? 4 : (c->deg_bucket <= 4 ? 3 : 2);
+        const int blocks = precision == HCFD_MIXED ? HCFD_FUSED_MIXED_BLOCKS : (c->deg_bucket <= 4 ? 3 : 2);
         const size_t budget = (size_t)160 * 1024 / blocks - 512;
         build_halo_tables(c->Nint, c->NcPad, c->deg_bucket, c->m.cnb, 256, precision == HCFD_MIXED, budget, c->halo,
                           c->st);

@@ -286,13 +286,18 @@ __device__ __forceinline__ void cfl_tile_lds(const PassArgs& a, int tile, int tb
 }
 
 // Minimum waves per SIMD for the single-kernel step (bounds its VGPRs; the
-// LDS of a tile allows about three blocks of four waves per CU).
+// LDS of a tile allows about three FP64 blocks of four waves per CU, four in
+// FP32-mixed; octagon meshes two).
 #ifndef HCFD_WAVES_FUSED
 #define HCFD_WAVES_FUSED 3
 #endif
+#ifndef HCFD_WAVES_FUSED_MIXED
+#define HCFD_WAVES_FUSED_MIXED 3
+#endif
 
 template <typename T, int W, int MAXDEG, bool CFL, bool MULTI>
-__global__ __launch_bounds__(FUSED_TILE) __attribute__((amdgpu_waves_per_eu(MAXDEG <= 3 ? HCFD_WAVES_FUSED : 2))) void
+__global__ __launch_bounds__(FUSED_TILE) __attribute__((amdgpu_waves_per_eu(
+    MAXDEG > 3 ? 2 : (sizeof(T) == 4 ? HCFD_WAVES_FUSED_MIXED : HCFD_WAVES_FUSED)))) void
 k_fused_step(
     PassArgs a) {
     using FT = FusedTypes<T>;
