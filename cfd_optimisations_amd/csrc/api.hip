@@ -443,6 +443,7 @@ Launch make_launch(hcfd_ctx* c, bool cfl, int cur, int pass) {
     a.FS = c->FS;
     a.FS32 = c->FS32;
     a.inst_off = c->m.inst_int_off;
+    a.chunk_inst = c->m.chunk_inst;
     a.pmin_first = c->pmin_first;
     a.pmin_last = c->pmin_last;
     a.dtacc = c->dtacc;

@@ -153,6 +153,14 @@ __device__ __forceinline__ double block_min(double x, double* sh) {
     return r;
 }
 
+// Instance of interior cell i: from the instance of its 64-cell chunk's first
+// cell, a forward scan over the (at most few) instance starts inside the chunk.
+__device__ __forceinline__ int instance_from(const int* __restrict__ off, const int* __restrict__ chunk_inst, int i) {
+    int b = chunk_inst[i >> 6];
+    while (off[b + 1] <= i) b++;
+    return b;
+}
+
 // Interior-cell range of instance b is [off[b], off[b+1]); returns b for cell i.
 __device__ __forceinline__ int instance_of(const int* __restrict__ off, int ninst, int i) {
     int lo = 0, hi = ninst - 1;

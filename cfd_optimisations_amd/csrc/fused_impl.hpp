@@ -259,11 +259,11 @@ __device__ __forceinline__ void cfl_tile_lds(const PassArgs& a, int tile, int tb
     if constexpr (MULTI) {
         const int c0 = tb;
         const int c1 = tb + tc - 1;
-        const int bf = instance_of(a.inst_off, a.ninst, c0);
-        const int bl = instance_of(a.inst_off, a.ninst, c1);
+        const int bf = instance_from(a.inst_off, a.chunk_inst, c0);
+        const int bl = instance_from(a.inst_off, a.chunk_inst, c1);
         vf = INFINITY;
         if (i < a.Nint) {
-            const int bi = instance_of(a.inst_off, a.ninst, i);
+            const int bi = instance_from(a.inst_off, a.chunk_inst, i);
             if (bi == bf) vf = dti;
             else if (bi == bl) vl = dti;
             else atomicMin(&a.dtacc[bi], dbits(dti));
@@ -379,7 +379,7 @@ k_fused_step(
         double dti = INFINITY;
         if (hasT) {
             const long long P = a.NcPad;
-            const int inst = MULTI ? instance_of(a.inst_off, a.ninst, i) : 0;
+            const int inst = MULTI ? instance_from(a.inst_off, a.chunk_inst, i) : 0;
             const double dt = a.dt[inst];
             double R[NV] = {0.0, 0.0, 0.0, 0.0};
             Kahan K[NV];

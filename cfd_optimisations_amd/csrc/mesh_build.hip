@@ -679,11 +679,11 @@ __global__ void k_max_reduce(int n, const int* __restrict__ a, int* __restrict__
 
 void free_device_mesh(DeviceMesh& m) {
     for (void* p : {(void*)m.slots, (void*)m.faces, (void*)m.cnb, (void*)m.cside, (void*)m.cgeo, (void*)m.iv,
-                    (void*)m.ghost_bc, (void*)m.inst_int_off, (void*)m.xf, (void*)m.xc})
+                    (void*)m.ghost_bc, (void*)m.inst_int_off, (void*)m.chunk_inst, (void*)m.xf, (void*)m.xc})
         if (p) (void)hipFree(p);
     m.slots = nullptr; m.faces = nullptr; m.cnb = nullptr; m.cside = nullptr; m.cgeo = nullptr; m.iv = nullptr;
     m.ghost_bc = nullptr;
-    m.inst_int_off = nullptr; m.xf = nullptr; m.xc = nullptr;
+    m.inst_int_off = nullptr; m.chunk_inst = nullptr; m.xf = nullptr; m.xc = nullptr;
 }
 
 void build_device_mesh(const BuildInput& in, DeviceMesh& out, hipStream_t st) {
@@ -928,6 +928,20 @@ void build_device_mesh(const BuildInput& in, DeviceMesh& out, hipStream_t st) {
     }
     out.inst_int_off = dalloc<int>(ninst + 1);
     h2d(out.inst_int_off, out.inst_int_off_h.data(), ninst + 1, st);
+    {
+        // instance of the first cell of every 64-cell chunk: kernels find a cell's
+        // instance from there with a short forward scan instead of a binary search
+        const int nch = std::max((out.inst_int_off_h[ninst] + 63) / 64, 1);
+        std::vector<int> ch(nch, 0);
+        int b = 0;
+        for (int c = 0; c < nch; c++) {
+            while (b + 1 < ninst && out.inst_int_off_h[b + 1] <= 64 * c) b++;
+            ch[c] = b;
+        }
+        out.chunk_inst = dalloc<int>(nch);
+        h2d(out.chunk_inst, ch.data(), nch, st);
+        HIPCHK(hipStreamSynchronize(st));
+    }
 
     // ---- face colouring ---------------------------------------------------------------
     int* d_colour = sc.get<int>(Nf);

@@ -42,6 +42,7 @@ struct DeviceMesh {
     double* iv = nullptr;           // [NcPad], new numbering, ghosts 0
     int* ghost_bc = nullptr;        // [max(Nghost,1)] BC table entry per ghost
     int* inst_int_off = nullptr;    // [ninst+1] interior ranges, new numbering
+    int* chunk_inst = nullptr;      // [ceil(Nint/64)] instance of cell 64c (batches: a start for inst_of)
     double* xf = nullptr;           // [2*Nf] caller order
     double* xc = nullptr;           // [2*Nc] caller order
     std::vector<int> new_of_old, old_of_new;     // cells

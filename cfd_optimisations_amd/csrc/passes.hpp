@@ -28,6 +28,7 @@ struct PassArgs {
     double* FS = nullptr;        // face states, FP64: [MAXDEG][2][NcPad] double2 planes
     float* FS32 = nullptr;       // face states, FP32-mixed: [MAXDEG][NcPad] float4
     const int* inst_off = nullptr;      // [ninst+1] interior ranges
+    const int* chunk_inst = nullptr;    // [ceil(Nint/64)] instance of cell 64c
     double* pmin_first = nullptr;       // [ntiles] per-tile CFL partial of the tile's first instance
     double* pmin_last = nullptr;        // [ntiles] ... of its last instance
     unsigned long long* dtacc = nullptr;// [ninst] CFL minimum (bit pattern) of cells in neither

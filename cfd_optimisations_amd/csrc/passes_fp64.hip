@@ -99,8 +99,8 @@ __global__ __launch_bounds__(1024) void k_dt_reduce(PassArgs a, double cfl, doub
             for (int tile = t0 + (int)threadIdx.x; tile <= t1; tile += blockDim.x) {
                 const int f0 = a.tile_start ? a.tile_start[tile] : tile * a.tile_cells;
                 const int l0 = a.tile_start ? a.tile_start[tile + 1] - 1 : min(f0 + a.tile_cells, a.Nint) - 1;
-                const int bf = instance_of(a.inst_off, a.ninst, f0);
-                const int bl = instance_of(a.inst_off, a.ninst, l0);
+                const int bf = instance_from(a.inst_off, a.chunk_inst, f0);
+                const int bl = instance_from(a.inst_off, a.chunk_inst, l0);
                 if (bf == b) m = fmin(m, a.pmin_first[tile]);
                 else if (bl == b) m = fmin(m, a.pmin_last[tile]);
             }

@@ -83,11 +83,11 @@ __device__ __forceinline__ void cfl_tile_range(const PassArgs& a, int tile, int 
         const double m = block_min(dti, sh);
         if (threadIdx.x == 0) a.pmin_first[tile] = m;
     } else {
-        const int bf = instance_of(a.inst_off, a.ninst, c0);
-        const int bl = instance_of(a.inst_off, a.ninst, c1);
+        const int bf = instance_from(a.inst_off, a.chunk_inst, c0);
+        const int bl = instance_from(a.inst_off, a.chunk_inst, c1);
         double vf = INFINITY, vl = INFINITY;
         if (i < a.Nint) {
-            const int bi = instance_of(a.inst_off, a.ninst, i);
+            const int bi = instance_from(a.inst_off, a.chunk_inst, i);
             if (bi == bf) vf = dti;
             else if (bi == bl) vl = dti;
             else atomicMin(&a.dtacc[bi], dbits(dti));
@@ -612,7 +612,7 @@ __global__ __launch_bounds__(256) HCFD_WAVES_B_ATTR void k_pass_b(PassArgs a) {
         __syncthreads();
         if (i >= a.Nint) continue;
         const int tc = min(a.tile_cells, a.Nc - tb);
-        const int inst = MULTI ? instance_of(a.inst_off, a.ninst, i) : 0;
+        const int inst = MULTI ? instance_from(a.inst_off, a.chunk_inst, i) : 0;
         if constexpr (MIXED) pass_b_cell_mixed<W, MAXDEG>(a, i, a.dt[inst], inst, sF, tb, tc);
         else pass_b_cell_fp64<W, MAXDEG>(a, i, a.dt[inst], inst, sF, tb, tc);
     }
