@@ -339,6 +339,25 @@ k_fused_step(
             load_cell_in<T, MAXDEG>(a, i, ci);
             load_cell<W, NV>(a.U, i, uown);
         }
+        // the halo ring goes to one wavefront, rotated from tile to tile so the
+        // extra round does not always land on the same SIMD; in FP32 (where the
+        // registers allow it without spilling) the inputs of its first ring cell
+        // are in flight during the staging as well
+        constexpr bool early_ring = sizeof(T) == 4;
+        const int rtid = (tid + WAVE * ((blockIdx.x + tile) & (FUSED_TILE / WAVE - 1))) & (FUSED_TILE - 1);
+        const int qh = FUSED_TILE - 1 - rtid;
+        CellIn<T, MAXDEG> ch;
+        if (early_ring && qh < L.nh1) load_cell_in<T, MAXDEG>(a, a.h1[h0 + qh], ch);
+        // staged halo cell q = tid (ring H1, then S2): its id and state are in
+        // flight before the barrier too; further rounds (more than 256 staged
+        // halo cells) load after it
+        const int nhalo = L.nh1 + L.ns2;
+        int gq = 0;
+        double uq[NV] = {0, 0, 0, 0};
+        if (tid < nhalo) {
+            gq = tid < L.nh1 ? a.h1[h0 + tid] : a.s2[s0 + tid - L.nh1];
+            load_cell<W, NV>(a.U, gq, uq);
+        }
         __syncthreads();  // the previous tile's readers are done
         if (hasT) {
             T ut[NV];
@@ -346,13 +365,18 @@ k_fused_step(
             for (int v = 0; v < NV; v++) ut[v] = (T)uown[v];
             FT::put(sU, su_pitch, tid, ut);
         }
-        for (int q = tid; q < L.nh1; q += FUSED_TILE) sH1[q] = a.h1[h0 + q];
-        for (int q = tid; q < L.ns2; q += FUSED_TILE) sS2[q] = a.s2[s0 + q];
-        const int nhalo = L.nh1 + L.ns2;
         for (int q = tid; q < nhalo; q += FUSED_TILE) {
-            const int g = q < L.nh1 ? a.h1[h0 + q] : a.s2[s0 + q - L.nh1];
+            int g = gq;
             double ud[NV];
-            load_cell<W, NV>(a.U, g, ud);
+            if (q == tid) {
+#pragma unroll
+                for (int v = 0; v < NV; v++) ud[v] = uq[v];
+            } else {
+                g = q < L.nh1 ? a.h1[h0 + q] : a.s2[s0 + q - L.nh1];
+                load_cell<W, NV>(a.U, g, ud);
+            }
+            if (q < L.nh1) sH1[q] = g;
+            else sS2[q - L.nh1] = g;
             T ut[NV];
 #pragma unroll
             for (int v = 0; v < NV; v++) ut[v] = (T)ud[v];
@@ -365,12 +389,8 @@ k_fused_step(
 #pragma unroll
         for (int k = 0; k < MAXDEG; k++) lnb[k] = -1;
         if (hasT) fused_cell_a<T, MAXDEG>(a, L, ci, tid, true, sU, su_pitch, sFS, fs_pitch, lnb, bad);
-        // the halo ring goes to one wavefront, rotated from tile to tile so the
-        // extra round does not always land on the same SIMD
-        const int rtid = (tid + WAVE * ((blockIdx.x + tile) & (FUSED_TILE / WAVE - 1))) & (FUSED_TILE - 1);
-        for (int q = FUSED_TILE - 1 - rtid; q < L.nh1; q += FUSED_TILE) {
-            CellIn<T, MAXDEG> ch;
-            load_cell_in<T, MAXDEG>(a, a.h1[h0 + q], ch);
+        for (int q = qh; q < L.nh1; q += FUSED_TILE) {
+            if (!early_ring || q != qh) load_cell_in<T, MAXDEG>(a, a.h1[h0 + q], ch);
             int ln[MAXDEG];
             fused_cell_a<T, MAXDEG>(a, L, ch, L.tc + q, false, sU, su_pitch, sFS, fs_pitch, ln, bad);
         }
