@@ -1,7 +1,11 @@
 #!/usr/bin/env python3
 """Benchmark of the explicit unstructured finite-volume Euler residual on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c3|c3mixed|c4|c5] [--check N]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c3|c3mixed|c4|c5|c2rk|c2out] [--check N]
+
+c2 (default) is the 4M-triangle FP64 workload; c3/c3mixed the 16M-triangle ramp; c4/c5
+the batches; c2rk the SSP-RK integrators' cost; c2out the output path (files written
+while stepping).
 
 A step is one residual evaluation plus the forward-Euler update (src/Base/Euler.cpp:85-97):
 pass A (gradient + limiter + CFL bound), the CFL reduction, pass B (flux + update +
@@ -243,6 +247,100 @@ def secondary_c3(steps=None):
     return out
 
 
+def run_integrators(steps: int):
+    """C2's mesh and state stepped with forward Euler, SSP-RK2 and SSP-RK3 (fixed dt,
+    the CFL step of the initial state / stages so the simulated time per stage
+    matches): time per step and per stage, and cell-updates/s counted per stage."""
+    from cfd_optimisations_amd import cases as C
+    from cfd_optimisations_amd import lib as L
+    w = C.WORKLOADS["c2_riemann_fp64"]
+    case = C.riemann_case(w["nx"], w["ny"], seed=0)
+    ctx = C.make_context(case, L.FP64, L.FLAG_ORDER_SFC)
+    inf = ctx.info()
+    U0 = ctx.get_state()
+    ctx.step(1, cfl=w["cfl"])
+    dt = ctx.last_dt()
+    rows = []
+    for stages in (1, 2, 3):
+        ctx.set_state(U0)
+        ctx.set_integrator(stages)
+        ctx.step(3, dt=dt)
+        tm = ctx.time_steps(steps, dt=dt)
+        ms = tm["wall_ms"] / steps
+        rows.append({"stages": stages, "ms_per_step": ms, "ms_per_stage": ms / stages,
+                     "stage_updates_per_s": inf.Nint * stages / (ms * 1e-3), "fault": ctx.fault()})
+        log(f"stages {stages}: {ms:.4f} ms per step")
+    sched = ctx.schedule()
+    ctx.close()
+    return {"metric": METRIC + " (integrator cost)", "value": rows[2]["stage_updates_per_s"],
+            "unit": "cell-updates/s (per stage, SSP-RK3)", "n_gpus": 1, "steps": steps, "warmup": 3,
+            "ms_per_step": rows[2]["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"c2 mesh ({w['nx']}x{w['ny']} quads, 4M triangles), fixed dt, FP64, "
+                                   f"forward Euler / SSP-RK2 / SSP-RK3", "schedule": sched, "dt": dt},
+            "integrators": rows}
+
+
+def run_output_overlap(steps: int, every: int):
+    """C2 with primitives written to an .hsln file every `every` steps: stepping
+    alone, the reference's pattern (stop, compute primitives, copy, write), and
+    snapshots taken on the device and copied on a second stream while stepping
+    continues, the file written by the host meanwhile."""
+    import numpy as np
+    from cfd_optimisations_amd import cases as C
+    from cfd_optimisations_amd import lib as L
+    w = C.WORKLOADS["c2_riemann_fp64"]
+    case = C.riemann_case(w["nx"], w["ny"], seed=0)
+    ctx = C.make_context(case, L.FP64, L.FLAG_ORDER_SFC)
+    inf = ctx.info()
+    cfl = w["cfl"]
+    out_dir = tempfile.mkdtemp(prefix="hcfd_out_")
+    U0 = ctx.get_state()
+    nout = steps // every
+    res = {}
+    ctx.step(5, cfl=cfl)
+    for mode in ("none", "sync", "overlap"):
+        ctx.set_state(U0)
+        ctx.sync()
+        t0 = time.perf_counter()
+        if mode == "none":
+            ctx.step(steps, cfl=cfl)
+        elif mode == "sync":
+            for k in range(nout):
+                ctx.step(every, cfl=cfl)
+                L.write_hsln(os.path.join(out_dir, f"sync_{k}.hsln"), ctx.get_prims())
+        else:
+            pending = None
+            for k in range(nout):
+                ctx.step_async(every, cfl=cfl)
+                slot = k & 1
+                ctx.output_begin(slot)
+                if pending is not None:
+                    Q, _ = ctx.output_wait(pending)
+                    L.write_hsln(os.path.join(out_dir, f"overlap_{k - 1}.hsln"), Q)
+                pending = slot
+            Q, _ = ctx.output_wait(pending)
+            L.write_hsln(os.path.join(out_dir, f"overlap_{nout - 1}.hsln"), Q)
+            ctx.sync()
+        res[mode] = time.perf_counter() - t0
+        log(f"output mode {mode}: {res[mode]:.3f} s")
+    same = all(np.array_equal(L.read_hsln(os.path.join(out_dir, f"sync_{k}.hsln")),
+                              L.read_hsln(os.path.join(out_dir, f"overlap_{k}.hsln"))) for k in range(nout))
+    for f in os.listdir(out_dir):
+        os.remove(os.path.join(out_dir, f))
+    os.rmdir(out_dir)
+    ctx.close()
+    return {"metric": METRIC + " (output path)", "value": inf.Nint * steps / res["overlap"],
+            "unit": "cell-updates/s (with output)", "n_gpus": 1, "steps": steps, "warmup": 5,
+            "ms_per_step": 1e3 * res["overlap"] / steps, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"c2 ({w['nx']}x{w['ny']} quads, 4M triangles), FP64, CFL {cfl}, "
+                                   f"{steps} steps, .hsln of all cells every {every} steps ({nout} files of "
+                                   f"{32 * inf.Nc / 1e6:.0f} MB)"},
+            "wall_s": res, "output_overhead_sync": res["sync"] / res["none"] - 1.0,
+            "output_overhead_overlap": res["overlap"] / res["none"] - 1.0, "files_identical": same}
+
+
 def log(msg: str) -> None:
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
@@ -396,7 +494,7 @@ def main(argv=None) -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=0, help="timed steps (default: the workload's)")
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c3mixed", "c4", "c5"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c3mixed", "c4", "c5", "c2rk", "c2out"])
     ap.add_argument("--check", type=int, default=0, help="batches: smallest instances checked against the oracle")
     ap.add_argument("--no-compare", action="store_true", help="c5: skip the FP64 batched comparison run")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample length (0 = skip)")
@@ -411,7 +509,11 @@ def main(argv=None) -> int:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus}).",
               file=sys.stderr)
         return 2
-    if a.workload in ("c4", "c5"):
+    if a.workload == "c2rk":
+        line = run_integrators(a.steps or 100)
+    elif a.workload == "c2out":
+        line = run_output_overlap(a.steps or 500, 50)
+    elif a.workload in ("c4", "c5"):
         line = run_batch(a.workload, a.steps, a.warmup, not a.no_tune, a.check, not a.no_compare)
     else:
         line = run_single(a.workload, a.steps, a.warmup, a.cpu_seconds, a.graph, not a.no_secondary, a.tune,

@@ -333,13 +333,10 @@ hcfd_ctx* create_ctx(int n, const hcfd_mesh_view* views, int precision, int flag
         HIPCHK(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
         // runnable when the largest tile's halo fits in LDS with two blocks per CU
         c->fused_ok = c->fused_lds <= (size_t)std::min(lim, 80 * 1024);
-        // default only in FP64 and for compact tiles (at most 5% of the tiles had to be
-        // split): strip-shaped tiles, as a reverse Cuthill-McKee order makes, recompute
-        // more halo than they update; Hilbert tiles keep the halo near a quarter.  In
-        // FP32-mixed the halo recompute costs about what the face-state round trip
-        // saves (unpacked FP32 issues at the FP64 rate), so the two passes stay the
-        // default there and the autotuner may still pick the single kernel
-        c->fused_default = precision != HCFD_MIXED && c->fused_ok && c->halo.split_tiles * 20 <= c->halo.ntiles;
+        // default only for compact tiles (at most 5% of the tiles had to be split):
+        // strip-shaped tiles, as a reverse Cuthill-McKee order makes, recompute more
+        // halo than they update; Hilbert tiles keep the halo near a quarter
+        c->fused_default = c->fused_ok && c->halo.split_tiles * 20 <= c->halo.ntiles;
     }
     c->sched.fused = c->fused_default ? 1 : 0;
     HIPCHK(hipStreamSynchronize(c->st));

@@ -111,7 +111,7 @@ typedef struct hcfd_info {
     long long device_bytes;
     hcfd_schedule sched;
     int fused_available;  /* the halo tiles of this mesh fit in LDS (see hcfd_schedule.fused); the
-                             single-kernel step is the default FP64 schedule when, in addition,
+                             single-kernel step is the default schedule when, in addition,
                              at most 5% of the 256-cell tiles had to be split to fit */
     int fused_lds_bytes;  /* LDS per block of the single-kernel step */
     int halo_max_cells;   /* most cells one tile stages: its own (at most 256) + halo ring + their neighbours */

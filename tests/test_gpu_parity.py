@@ -2,7 +2,7 @@
 
 Every FP64 schedule (two passes per step, or one kernel per step over halo
 tiles) must be bit-identical to the oracle, which is itself bit-identical to
-the reference's Base sources built with -ffp-contract=off
+the reference's Base sources built with -ffp-contract=off; the stated bound
 of 1e-12 normwise relative per variable is checked as well.  The mixed path
 must stay within its stated bound of the FP64 path.
 """
@@ -18,7 +18,7 @@ from conftest import GOLDEN
 
 pytestmark = pytest.mark.gpu
 
-FP64_TOL = 1e-12   # north-star FP64 bound (normwise relative, per variable)
+FP64_TOL = 1e-12   # stated FP64 bound (normwise relative, per variable)
 
 # the two step paths: two passes per step, and one kernel per step over halo tiles
 TWO_PASS = dict(block=256, lanes_per_cell=1, aosoa_width=1, persistent=0, xcd_remap=1, fused=0)
@@ -310,7 +310,7 @@ def test_batch_faulting_instance_is_isolated(sched):
             assert np.array_equal(b.inst_state(k), Uk), k
 
 
-MIXED_RHO_TOL = 1e-5   # north-star bound on the relative L2 density error vs FP64
+MIXED_RHO_TOL = 1e-5   # stated bound on the mixed path's relative L2 density error vs FP64
 
 
 def test_mixed_fused_equals_two_pass():
