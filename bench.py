@@ -166,8 +166,8 @@ def run_single(name: str, steps: int, warmup: int, cpu_s: float, graph: bool, se
     if tune:
         ctx.autotune(steps=10, cfl=cfl, cache=os.path.join(tempfile.gettempdir(), "hcfd_tune.txt"))
     ctx.step(warmup, cfl=cfl)
-    # timed region: exactly `steps` steps (hipGraph replays unless --no-graph),
-    # the stream drained on both sides
+    # timed region: exactly `steps` steps (hipGraph replays with --graph), the
+    # stream drained on both sides
     ctx.sync()
     t0 = time.perf_counter()
     ctx.step(steps, cfl=cfl)
@@ -503,7 +503,8 @@ def main(argv=None) -> int:
     ap.add_argument("--check", type=int, default=0, help="batches: smallest instances checked against the oracle")
     ap.add_argument("--no-compare", action="store_true", help="c5: skip the FP64 batched comparison run")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample length (0 = skip)")
-    ap.add_argument("--no-graph", action="store_true", help="single meshes: eager launches in the timed region")
+    ap.add_argument("--graph", action="store_true", help="single meshes: replay a captured step graph in the "
+                    "timed region (measured no faster than eager launches on C2)")
     ap.add_argument("--order", default="sfc", choices=["sfc", "rcm"],
                     help="interior cell order: Hilbert curve of centroids (sfc) or reverse Cuthill-McKee (rcm)")
     ap.add_argument("--tune", action="store_true", help="run the autotuner before timing")
@@ -521,7 +522,7 @@ def main(argv=None) -> int:
     elif a.workload in ("c4", "c5"):
         line = run_batch(a.workload, a.steps, a.warmup, not a.no_tune, a.check, not a.no_compare)
     else:
-        line = run_single(a.workload, a.steps, a.warmup, a.cpu_seconds, not a.no_graph, not a.no_secondary, a.tune,
+        line = run_single(a.workload, a.steps, a.warmup, a.cpu_seconds, a.graph, not a.no_secondary, a.tune,
                           a.order)
     print(json.dumps(line))
     return 0
