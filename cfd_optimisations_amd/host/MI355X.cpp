@@ -222,6 +222,20 @@ void Euler::gather_BCs(const BCtype* BCmap, const double QBC[][NVARS]) {
 }
 
 // Euler::initialise (src/Base/Euler.cpp:62-77), on the device.
+void Euler::set_stages(int stages) { check(hcfd_set_integrator(domain.ctx(), stages), "hcfd_set_integrator"); }
+
+void Euler::restart(const std::string& hsln_path) {
+    int n = 0;
+    check(hcfd_read_hsln(hsln_path.c_str(), &n, nullptr), "hcfd_read_hsln");
+    if (n != Nc)
+        throw std::runtime_error("restart: " + hsln_path + " holds " + std::to_string(n) + " cells, the mesh " +
+                                 std::to_string(Nc));
+    check(hcfd_read_hsln(hsln_path.c_str(), &n, Q), "hcfd_read_hsln");
+    const double* const Qc[NVARS] = {Q[0], Q[1], Q[2], Q[3]};
+    check(hcfd_initialise_prims(domain.ctx(), Qc), "hcfd_initialise_prims");
+    check(hcfd_get_state(domain.ctx(), U), "hcfd_get_state");
+}
+
 void Euler::initialise(const double QIC[NVARS]) {
     check(hcfd_initialise(domain.ctx(), QIC), "hcfd_initialise");
     check(hcfd_get_state(domain.ctx(), U), "hcfd_get_state");

@@ -27,6 +27,12 @@ class Euler {
 
     // Output file stem (the reference writes ../sols/<Layout>, src/Base/Euler.cpp:22).
     void set_output(const std::string& fname);
+    // Stages per step of run(): 1 forward Euler (the reference's), 2 SSP-RK2,
+    // 3 SSP-RK3 (no reference counterpart; hcfd_set_integrator).
+    void set_stages(int stages);
+    // Start from a .hsln file written by run() (primitives of all cells) instead
+    // of initialise(); the file must belong to this mesh.
+    void restart(const std::string& hsln_path);
     // Conserved state copied back from the GPU into U.
     const double* const* state();
 

@@ -1,18 +1,22 @@
 // hcfd_run.cpp -- the reference's Test.cpp case (src/Test.cpp:50-74) driven
 // through the MI355X drop-in classes:
 //
-//   hcfd_run [mesh.hmsh] [output stem] [steps] [--stages]
+//   hcfd_run [mesh.hmsh] [output stem] [steps] [--stages] [--outputs K] [--rk S]
+//            [--restart file.hsln]
 //
 // Default: ../wedge.hmsh, ../sols/MI355X, 1000 steps of dt = 1e-3, writing
 // <stem>_0.hsln and <stem>_1.hsln as the reference writes sols/Base_{0,1}.hsln.
 // --stages runs the steps through the per-stage classes on host arrays
 // (GaussGreen, Limiter, BC copies, Flux, update, BC cells) instead of the
-// fused device step.
+// fused device step.  --outputs K writes K files after the initial one
+// (dt_out = T / K); --rk S steps with S-stage SSP-RK; --restart starts from a
+// .hsln file instead of the uniform initial state.
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <exception>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -29,7 +33,17 @@ int main(int argc, char** argv) {
         const char* mesh_path = argc > 1 ? argv[1] : "../wedge.hmsh";
         const std::string out = argc > 2 ? argv[2] : "../sols/MI355X";
         const int niter = argc > 3 ? std::atoi(argv[3]) : 1000;
-        const bool stages = argc > 4 && std::strcmp(argv[4], "--stages") == 0;
+        bool stages = false;
+        int nout = 1, rk = 1;
+        std::string restart;
+        for (int k = 4; k < argc; k++) {
+            if (std::strcmp(argv[k], "--stages") == 0) stages = true;
+            else if (std::strcmp(argv[k], "--outputs") == 0 && k + 1 < argc) nout = std::atoi(argv[++k]);
+            else if (std::strcmp(argv[k], "--rk") == 0 && k + 1 < argc) rk = std::atoi(argv[++k]);
+            else if (std::strcmp(argv[k], "--restart") == 0 && k + 1 < argc) restart = argv[++k];
+            else throw std::runtime_error(std::string("unknown option ") + argv[k]);
+        }
+        if (nout < 1) throw std::runtime_error("--outputs must be at least 1");
 
         // the case of src/Test.cpp:51-64
         const int nvars = 4;
@@ -51,10 +65,12 @@ int main(int argc, char** argv) {
         intg.set_output(out);
         intg.gather_BCs(BCmap, QBC);
         intg.initialise(QIC);
+        if (!restart.empty()) intg.restart(restart);
+        intg.set_stages(rk);
 
         auto t1 = std::chrono::steady_clock::now();
         if (!stages) {
-            intg.run(niter * dt, dt, niter * dt);
+            intg.run(niter * dt, dt, niter * dt / nout);
         } else {
             // Euler::run's loop body (src/Base/Euler.cpp:85-97) stage by stage
             const int Nc = domain.Nc;

@@ -51,3 +51,67 @@ def test_dropin_driver_reports_bad_mesh(tmp_path):
     bad.write_bytes(b"\x02\x00\x00\x00" * 3)
     r = subprocess.run([RUN, str(bad), str(tmp_path / "x"), "1"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "truncated" in r.stderr
+
+
+def _run(args):
+    r = subprocess.run([RUN] + [str(a) for a in args], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    return r
+
+
+def _output_steps(T, dt, dt_out):
+    """Steps after which Euler::run writes a file (src/Base/Euler.cpp:83-104)."""
+    t, t_out, n, out = 0.0, dt_out, 0, []
+    while t < T:
+        t += dt
+        n += 1
+        if t >= t_out:
+            out.append(n)
+            t_out += dt_out
+    return out
+
+
+def test_dropin_outputs_written_while_stepping(tmp_path, wedge_mesh):
+    """Four outputs over 400 steps: each file, written by the host while the device
+    steps on, holds the oracle's primitives of its own step."""
+    stem = str(tmp_path / "out")
+    _run([os.path.join(GOLDEN, "wedge.hmsh"), stem, 400, "--outputs", 4])
+    case = C.reference_wedge_case(wedge_mesh)
+    od = ol.OracleDomain(wedge_mesh, case.gamma, case.bctypes, case.qbc)
+    od.zero_bc_iv()
+    U = od.initialise(case.qic)
+    _, S0 = read_hsln(stem + "_0.hsln")
+    assert np.array_equal(S0, od.prims(U))
+    done = 0
+    steps = _output_steps(400 * 1e-3, 1e-3, 400 * 1e-3 / 4)
+    assert len(steps) == 4
+    for k, n in enumerate(steps, start=1):
+        U, _ = od.run(U, n - done, dt=1e-3)
+        done = n
+        _, Sk = read_hsln(f"{stem}_{k}.hsln")
+        assert np.array_equal(Sk, od.prims(U)), k
+    assert not os.path.exists(f"{stem}_5.hsln")
+
+
+def test_dropin_ssprk_and_restart(tmp_path, wedge_mesh):
+    """SSP-RK3 through the drop-in Euler, then a restart from its output file:
+    both equal the oracle (the restart starts from the file's primitives)."""
+    stem = str(tmp_path / "rk")
+    _run([os.path.join(GOLDEN, "wedge.hmsh"), stem, 150, "--rk", 3])
+    case = C.reference_wedge_case(wedge_mesh)
+    od = ol.OracleDomain(wedge_mesh, case.gamma, case.bctypes, case.qbc)
+    od.zero_bc_iv()
+    U, _ = od.run_ssprk(od.initialise(case.qic), 150, 3, dt=1e-3)
+    _, S1 = read_hsln(stem + "_1.hsln")
+    assert np.array_equal(S1, od.prims(U))
+    stem2 = str(tmp_path / "again")
+    _run([os.path.join(GOLDEN, "wedge.hmsh"), stem2, 100, "--restart", stem + "_1.hsln"])
+    _, R0 = read_hsln(stem2 + "_0.hsln")
+    U2 = od.initialise_prims(S1)
+    assert np.array_equal(R0, od.prims(U2))
+    U3, _ = od.run(U2, 100, dt=1e-3)
+    _, R1 = read_hsln(stem2 + "_1.hsln")
+    assert np.array_equal(R1, od.prims(U3))
+    r = subprocess.run([RUN, os.path.join(GOLDEN, "wedge.hmsh"), str(tmp_path / "x"), "1", "--restart",
+                        str(tmp_path / "missing.hsln")], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "cannot open" in r.stderr
