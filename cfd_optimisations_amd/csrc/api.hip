@@ -1012,10 +1012,11 @@ int hcfd_output_begin(hcfd_ctx* c, int slot) {
         if (c->out_pending[slot]) fail(HCFD_ERR_STATE, "output slot still pending: hcfd_output_wait first");
         const size_t n = 4 * (size_t)c->Nc + 1;  // primitives [4][Nc], then the clock
         if (!c->cst) HIPCHK(hipStreamCreateWithFlags(&c->cst, hipStreamNonBlocking));
-        if (!c->out_dev[slot]) {
-            c->out_dev[slot] = dmalloc<double>(n);
-            HIPCHK(hipEventCreateWithFlags(&c->out_ready[slot], hipEventDisableTiming));
-            HIPCHK(hipEventCreateWithFlags(&c->out_copied[slot], hipEventDisableTiming));
+        if (!c->out_host[slot]) {
+            // all four resources or none: a failure leaves the slot unallocated
+            if (!c->out_dev[slot]) c->out_dev[slot] = dmalloc<double>(n);
+            if (!c->out_ready[slot]) HIPCHK(hipEventCreateWithFlags(&c->out_ready[slot], hipEventDisableTiming));
+            if (!c->out_copied[slot]) HIPCHK(hipEventCreateWithFlags(&c->out_copied[slot], hipEventDisableTiming));
             void* h = nullptr;
             if (hipHostMalloc(&h, sizeof(double) * n, hipHostMallocDefault) != hipSuccess || !h) {
                 (void)hipGetLastError();
