@@ -372,3 +372,32 @@ def test_every_bc_type_exact(flow, sched):
         ctx.stage_set(L.ARR_U, W)
         ctx.stage("bc_cells")
         assert np.array_equal(ctx.stage_get(L.ARR_U), od.bc_cells(W))
+
+
+def test_c1_wedge_configuration_exact():
+    """The reference's CPU-runnable configuration: a 15 deg wedge of about 50k
+    triangles, Mach 2.5, slip walls and supersonic in/outflow, CFL 0.5, 2,000
+    steps; bit-identical to the oracle, clock included."""
+    w = C.WORKLOADS["c1_wedge_cpu"]
+    case = C.ramp_case(w["nx"], w["ny"], w["mach"], w["angle"], seed=0)
+    od, U0 = oracle_for(case)
+    Uo, to = od.run(U0, w["steps"], cfl=w["cfl"])
+    with C.make_context(case, L.FP64, L.FLAG_ORDER_SFC) as ctx:
+        t = ctx.step(w["steps"], cfl=w["cfl"])
+        assert t == to
+        assert np.array_equal(ctx.get_state(), Uo)
+        assert ctx.fault() == 0
+
+
+def test_c2_benchmark_mesh_exact():
+    """The 4M-triangle benchmark mesh itself (cells shuffled, renumbered on the
+    device), default schedule: its first steps bit-identical to the oracle."""
+    w = C.WORKLOADS["c2_riemann_fp64"]
+    case = C.riemann_case(w["nx"], w["ny"], seed=0)
+    od, U0 = oracle_for(case)
+    Uo, to = od.run(U0, 2, cfl=w["cfl"])
+    with C.make_context(case, L.FP64, L.FLAG_ORDER_SFC) as ctx:
+        assert ctx.schedule()["fused"] == 1
+        t = ctx.step(2, cfl=w["cfl"])
+        assert t == to
+        assert np.array_equal(ctx.get_state(), Uo)
