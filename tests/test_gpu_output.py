@@ -82,3 +82,23 @@ def test_batch_fixed_dt_clocks():
         for _ in range(7):
             tt += 1e-4
         assert list(t) == [tt, tt] and list(d) == [1e-4, 1e-4] and not f.any()
+
+
+def test_batch_snapshot_is_concatenated_prims():
+    """A batch's snapshot holds every instance's primitives in the batch's caller
+    order (instances one after another), as hcfd_get_prims does."""
+    cs = [C.ramp_case(24, 12, 2.0, 10.0, seed=1), C.ramp_case(16, 8, 1.8, 5.0, seed=3)]
+    with C.make_batch(cs, L.FP64, L.FLAG_ORDER_SFC) as b:
+        b.step(5, cfl=0.5)
+        b.output_begin(1)
+        b.step_async(5, cfl=0.5)
+        Q, t = b.output_wait(1)
+        b.sync()
+        off = 0
+        for k, c in enumerate(cs):
+            od, U0 = oracle_for(c)
+            U5, t5 = od.run(U0, 5, cfl=0.5)
+            n = U0.shape[1]
+            assert np.array_equal(Q[:, off:off + n], od.prims(U5)), k
+            off += n
+        assert off == Q.shape[1]

@@ -401,3 +401,24 @@ def test_c2_benchmark_mesh_exact():
         t = ctx.step(2, cfl=w["cfl"])
         assert t == to
         assert np.array_equal(ctx.get_state(), Uo)
+
+
+def test_mixed_quads_wedge(wedge_mesh):
+    """FP32-mixed on the quadrilateral wedge (four slots per cell): the single
+    kernel equals the two passes bit for bit and stays within the stated bound
+    of the FP64 path after src/Test.cpp's 1000 steps."""
+    case = C.reference_wedge_case(wedge_mesh)
+    with C.make_context(case, L.FP64, L.FLAG_ORDER_SFC) as a, \
+            C.make_context(case, L.MIXED, L.FLAG_ORDER_SFC) as m, \
+            C.make_context(case, L.MIXED, L.FLAG_ORDER_SFC) as mf:
+        m.set_schedule(**TWO_PASS)
+        if not mf.info().fused_available:
+            pytest.skip("wedge tiles do not fit the single-kernel step")
+        mf.set_schedule(**FUSED)
+        for ctx in (a, m, mf):
+            ctx.step(1000, dt=1e-3)
+        Ua, Um, Uf = a.get_state(), m.get_state(), mf.get_state()
+        assert m.fault() == 0 and mf.fault() == 0
+    assert np.array_equal(Um, Uf)
+    e = relerr(Um, Ua)
+    assert e[0] < MIXED_RHO_TOL, e
