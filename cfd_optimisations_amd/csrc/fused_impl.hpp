@@ -250,30 +250,29 @@ struct TileMin {
     unsigned arrived;
 };
 
+// inst: the lane's cell's instance (-1 without a cell); bf, bl: the instances of
+// the tile's first and last cell.  A tile inside one instance (bf == bl, the
+// common case) needs only the first partial.
 template <bool MULTI>
-__device__ __forceinline__ void cfl_tile_lds(const PassArgs& a, int tile, int tb, int tc, int i, double dti,
+__device__ __forceinline__ void cfl_tile_lds(const PassArgs& a, int tile, int inst, int bf, int bl, double dti,
                                              TileMin* tm) {
     const int lane = threadIdx.x & (WAVE - 1);
     const int nwaves = FUSED_TILE / WAVE;
     double vf = dti, vl = INFINITY;
-    if constexpr (MULTI) {
-        const int c0 = tb;
-        const int c1 = tb + tc - 1;
-        const int bf = instance_from(a.inst_off, a.chunk_inst, c0);
-        const int bl = instance_from(a.inst_off, a.chunk_inst, c1);
+    const bool two = MULTI && bf != bl;  // block-uniform
+    if (two) {
         vf = INFINITY;
-        if (i < a.Nint) {
-            const int bi = instance_from(a.inst_off, a.chunk_inst, i);
-            if (bi == bf) vf = dti;
-            else if (bi == bl) vl = dti;
-            else atomicMin(&a.dtacc[bi], dbits(dti));
+        if (inst >= 0) {
+            if (inst == bf) vf = dti;
+            else if (inst == bl) vl = dti;
+            else atomicMin(&a.dtacc[inst], dbits(dti));
         }
         vl = wave_min(vl);
     }
     vf = wave_min(vf);
     if (lane == 0) {
         atomicMin(&tm->first, dbits(vf));
-        if constexpr (MULTI) atomicMin(&tm->last, dbits(vl));
+        if (two) atomicMin(&tm->last, dbits(vl));
         __threadfence_block();
         if (atomicAdd(&tm->arrived, 1u) == nwaves - 1) {
             __threadfence_block();
@@ -332,6 +331,12 @@ k_fused_step(
         L.sS2 = sS2;
         const int i = L.tb + tid;
         const bool hasT = tid < L.tc;
+        // instances of the tile's first and last cell (batches)
+        int bf = 0, bl = 0;
+        if constexpr (MULTI) {
+            bf = instance_from(a.inst_off, a.chunk_inst, L.tb);
+            bl = instance_from(a.inst_off, a.chunk_inst, L.tb + L.tc - 1);
+        }
         // this lane's tile cell: its inputs are in flight while the tile is staged
         CellIn<T, MAXDEG> ci;
         double uown[NV] = {0, 0, 0, 0};  // U^n of the lane's tile cell (FP64: read back from LDS in pass B)
@@ -397,9 +402,13 @@ k_fused_step(
         __syncthreads();
         // pass B over the tile
         double dti = INFINITY;
+        int lane_inst = -1;
         if (hasT) {
             const long long P = a.NcPad;
-            const int inst = MULTI ? instance_from(a.inst_off, a.chunk_inst, i) : 0;
+            int inst = bf;
+            if (MULTI && bf != bl)
+                while (a.inst_off[inst + 1] <= i) inst++;
+            lane_inst = inst;
             const double dt = a.dt[inst];
             double R[NV] = {0.0, 0.0, 0.0, 0.0};
             Kahan K[NV];
@@ -457,7 +466,7 @@ k_fused_step(
         } else if (bad) {
             a.fault[0] = 1;
         }
-        if constexpr (CFL) cfl_tile_lds<MULTI>(a, tile, L.tb, L.tc, hasT ? i : a.Nint, dti, tmin);
+        if constexpr (CFL) cfl_tile_lds<MULTI>(a, tile, lane_inst, bf, bl, dti, tmin);
     }
 }
 
