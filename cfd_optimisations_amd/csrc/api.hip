@@ -128,6 +128,10 @@ struct hcfd_ctx {
     ~hcfd_ctx();
 };
 
+// LDS allocation unit assumed when sizing a block's share of the 160 KB per CU.
+#ifndef HCFD_LDS_GRANULE
+#define HCFD_LDS_GRANULE 2048
+#endif
 // Blocks per CU the FP32-mixed single-kernel step's LDS budget is sized for.
 #ifndef HCFD_FUSED_MIXED_BLOCKS
 #define HCFD_FUSED_MIXED_BLOCKS 4
@@ -322,7 +326,9 @@ hcfd_ctx* create_ctx(int n, const hcfd_mesh_view* views, int precision, int flag
     // blocks of four wavefronts per CU (two for octagon meshes), FP32-mixed four
     {
         const int blocks = precision == HCFD_MIXED ? HCFD_FUSED_MIXED_BLOCKS : (c->deg_bucket <= 4 ? 3 : 2);
-        const size_t budget = (size_t)160 * 1024 / blocks - 512;
+        // the share is rounded down to whole HCFD_LDS_GRANULE units: measured, blocks
+        // of 53.9 KB (under a third of 160 KB) run two per CU, blocks of 53.5 KB three
+        const size_t budget = (size_t)160 * 1024 / blocks / HCFD_LDS_GRANULE * HCFD_LDS_GRANULE;
         build_halo_tables(c->Nint, c->NcPad, c->deg_bucket, c->m.cnb, 256, precision == HCFD_MIXED, budget, c->halo,
                           c->st);
     }
