@@ -422,3 +422,23 @@ def test_mixed_quads_wedge(wedge_mesh):
     assert np.array_equal(Um, Uf)
     e = relerr(Um, Ua)
     assert e[0] < MIXED_RHO_TOL, e
+
+
+@pytest.mark.parametrize("prec", [L.FP64, L.MIXED], ids=["fp64", "mixed"])
+def test_free_stream_preserved_on_16m_ramp(prec):
+    """Size-independent check on the 16M-triangle ramp mesh itself: a uniform
+    Mach 3 state with supersonic inflow on every boundary stays uniform (closed
+    cells, consistent flux) through 10 CFL steps, to rounding."""
+    import dataclasses
+    w = C.WORKLOADS["c3_ramp_16m"]
+    base = C.ramp_case(w["nx"], w["ny"], w["mach"], w["angle"], seed=0)
+    case = dataclasses.replace(base, bctypes=np.full(4, L.SUPINLET, np.int32))
+    with C.make_context(case, prec, L.FLAG_ORDER_SFC) as ctx:
+        U0 = ctx.get_state()
+        ctx.step(10, cfl=w["cfl"])
+        U = ctx.get_state()
+        assert ctx.fault() == 0
+    tol = 1e-12 if prec == L.FP64 else 2e-6
+    for v in range(4):
+        scale = np.abs(U0[v]).max()
+        assert np.abs(U[v] - U0[v]).max() <= tol * max(scale, 1.0), v
