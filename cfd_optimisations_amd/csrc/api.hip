@@ -310,8 +310,7 @@ hcfd_ctx* create_ctx(int n, const hcfd_mesh_view* views, int precision, int flag
     c->lastdt = dmalloc<double>(n);
     c->fault = dmalloc<int>(n);
     {
-        std::vector<unsigned long long> inf(n, (unsigned long long)0x7ff0000000000000ull);
-        HIPCHK(hipMemcpyAsync(c->dtacc, inf.data(), sizeof(unsigned long long) * n, hipMemcpyHostToDevice, c->st));
+        HIPCHK(hipMemsetAsync(c->dtacc, 0, sizeof(unsigned long long) * n, c->st));  // maximum rate: 0
         HIPCHK(hipMemsetAsync(c->tdev, 0, sizeof(double) * n, c->st));
         HIPCHK(hipMemsetAsync(c->lastdt, 0, sizeof(double) * n, c->st));
         HIPCHK(hipMemsetAsync(c->fault, 0, sizeof(int) * n, c->st));

@@ -131,6 +131,35 @@ __device__ __forceinline__ void store_cell(T* __restrict__ a, int i, const T in[
     }
 }
 
+// CFL rate of a cell, iv * sum_f A_f (|u.n_f| + c): the per-cell bound of the
+// step is 1/rate, so dt = cfl * min_i 1/rate_i = cfl * (1 / max_i rate_i)
+// exactly (a correctly rounded 1/x is monotone in x) and the kernels reduce the
+// maximum rate, leaving one divide per instance.  NaN (a faulted cell) maps to
+// 0, which the maximum ignores, as the minimum of bounds ignores NaN.
+__device__ __forceinline__ double cfl_rate(double x) { return x >= 0.0 ? x : 0.0; }
+
+// Wave-level maximum over 64 lanes (non-negative values).
+__device__ __forceinline__ double wave_max(double x) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x = fmax(x, __shfl_xor(x, off, 64));
+    return x;
+}
+
+// Block-level maximum; every thread receives the result.  `sh` holds one
+// double per wave of the block.
+__device__ __forceinline__ double block_max(double x, double* sh) {
+    x = wave_max(x);
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int w = threadIdx.x / WAVE;
+    const int nw = (blockDim.x + WAVE - 1) / WAVE;
+    __syncthreads();
+    if (lane == 0) sh[w] = x;
+    __syncthreads();
+    double r = sh[0];
+    for (int k = 1; k < nw; k++) r = fmax(r, sh[k]);
+    return r;
+}
+
 // Wave-level minimum over 64 lanes.
 __device__ __forceinline__ double wave_min(double x) {
 #pragma unroll

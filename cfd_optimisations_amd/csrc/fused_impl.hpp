@@ -214,7 +214,8 @@ __device__ __forceinline__ void fused_cell_a(const PassArgs& a, const TileLists&
     for (int k = 0; k < MAXDEG; k++) lnb_out[k] = ln[k];
 }
 
-// CFL bound 1 / (iv sum_f A_f (|u.n_f| + c)) of an FP64 state, from a cell's inputs.
+// CFL rate iv sum_f A_f (|u.n_f| + c) of an FP64 state, from a cell's inputs
+// (the cell's bound is 1/rate; see cfl_rate).
 template <typename T, int MAXDEG>
 __device__ __forceinline__ double cfl_bound_in(double gamma, const CellIn<T, MAXDEG>& ci, const double u[NV]) {
     const double gmo = gamma - 1;
@@ -228,7 +229,7 @@ __device__ __forceinline__ double cfl_bound_in(double gamma, const CellIn<T, MAX
         if (ci.nbx[k] == SLOT_EMPTY) continue;
         lam += (double)ci.A[k] * (fabs(ux * (double)ci.n0[k] + uy * (double)ci.n1[k]) + c);
     }
-    return 1.0 / ((double)ci.iv * lam);
+    return cfl_rate((double)ci.iv * lam);
 }
 
 template <typename T, int MAXDEG>
@@ -239,10 +240,10 @@ __device__ __forceinline__ double cfl_bound(const PassArgs& a, int g, const doub
 }
 
 // CFL partials of one tile without a block barrier: each wavefront folds its
-// minimum into LDS with an atomic, and the last of the tile's wavefronts to
+// maximum rate into LDS with an atomic, and the last of the tile's wavefronts to
 // arrive (LDS counter) publishes the tile partial and resets the slots.  The
-// bounds are positive (or +inf, or NaN for a faulted cell), so the bit patterns
-// order like the values and NaN never wins, as with fmin.  For batches the
+// rates are non-negative (or +inf; cfl_rate maps NaN to 0), so the bit patterns
+// order like the values.  For batches the
 // tile's first and last instance get their own slot; cells of instances
 // strictly inside the tile use global atomics, as in cfl_tile.
 struct TileMin {
@@ -254,31 +255,30 @@ struct TileMin {
 // the tile's first and last cell.  A tile inside one instance (bf == bl, the
 // common case) needs only the first partial.
 template <bool MULTI>
-__device__ __forceinline__ void cfl_tile_lds(const PassArgs& a, int tile, int inst, int bf, int bl, double dti,
+__device__ __forceinline__ void cfl_tile_lds(const PassArgs& a, int tile, int inst, int bf, int bl, double rate,
                                              TileMin* tm) {
     const int lane = threadIdx.x & (WAVE - 1);
     const int nwaves = FUSED_TILE / WAVE;
-    double vf = dti, vl = INFINITY;
+    double vf = rate, vl = 0.0;
     const bool two = MULTI && bf != bl;  // block-uniform
     if (two) {
-        vf = INFINITY;
+        vf = 0.0;
         if (inst >= 0) {
-            if (inst == bf) vf = dti;
-            else if (inst == bl) vl = dti;
-            else atomicMin(&a.dtacc[inst], dbits(dti));
+            if (inst == bf) vf = rate;
+            else if (inst == bl) vl = rate;
+            else atomicMax(&a.dtacc[inst], dbits(rate));
         }
-        vl = wave_min(vl);
+        vl = wave_max(vl);
     }
-    vf = wave_min(vf);
+    vf = wave_max(vf);
     if (lane == 0) {
-        atomicMin(&tm->first, dbits(vf));
-        if (two) atomicMin(&tm->last, dbits(vl));
+        atomicMax(&tm->first, dbits(vf));
+        if (two) atomicMax(&tm->last, dbits(vl));
         __threadfence_block();
         if (atomicAdd(&tm->arrived, 1u) == nwaves - 1) {
             __threadfence_block();
-            a.pmin_first[tile] = __longlong_as_double((long long)atomicExch(&tm->first, dbits(INFINITY)));
-            if constexpr (MULTI)
-                a.pmin_last[tile] = __longlong_as_double((long long)atomicExch(&tm->last, dbits(INFINITY)));
+            a.pmin_first[tile] = __longlong_as_double((long long)atomicExch(&tm->first, 0ull));
+            if constexpr (MULTI) a.pmin_last[tile] = __longlong_as_double((long long)atomicExch(&tm->last, 0ull));
             tm->arrived = 0;
         }
     }
@@ -310,8 +310,8 @@ k_fused_step(
     TileMin* tmin = reinterpret_cast<TileMin*>(sS2 + ((a.s2max + 3) & ~3));
     const int tid = threadIdx.x;
     if (tid == 0) {
-        tmin->first = dbits(INFINITY);
-        tmin->last = dbits(INFINITY);
+        tmin->first = 0ull;
+        tmin->last = 0ull;
         tmin->arrived = 0;
     }  // first read after the tile loop's first barrier
 
@@ -401,7 +401,7 @@ k_fused_step(
         }
         __syncthreads();
         // pass B over the tile
-        double dti = INFINITY;
+        double rate = 0.0;
         int lane_inst = -1;
         if (hasT) {
             const long long P = a.NcPad;
@@ -462,11 +462,11 @@ k_fused_step(
                     store_cell<W, NV>(a.Unext, gc, ug);
                 }
             }
-            if constexpr (CFL) dti = cfl_bound_in<T, MAXDEG>(a.gamma, ci, unew);
+            if constexpr (CFL) rate = cfl_bound_in<T, MAXDEG>(a.gamma, ci, unew);
         } else if (bad) {
             a.fault[0] = 1;
         }
-        if constexpr (CFL) cfl_tile_lds<MULTI>(a, tile, lane_inst, bf, bl, dti, tmin);
+        if constexpr (CFL) cfl_tile_lds<MULTI>(a, tile, lane_inst, bf, bl, rate, tmin);
     }
 }
 
@@ -480,13 +480,13 @@ __global__ __launch_bounds__(FUSED_TILE) void k_cfl_bound(PassArgs a) {
     for (int tile = tfirst; tile < tend; tile += step) {
         const int tb = a.tile_start[tile], tc = a.tile_start[tile + 1] - tb;
         const int i = tb + threadIdx.x;
-        double dti = INFINITY;
+        double rate = 0.0;
         if ((int)threadIdx.x < tc) {
             double u[NV];
             load_cell<W, NV>(a.U, i, u);
-            dti = cfl_bound<T, MAXDEG>(a, i, u);
+            rate = cfl_bound<T, MAXDEG>(a, i, u);
         }
-        cfl_tile_range<MULTI>(a, tile, tb, tb + tc - 1, (int)threadIdx.x < tc ? i : a.Nint, dti, sh);
+        cfl_tile_range<MULTI>(a, tile, tb, tb + tc - 1, (int)threadIdx.x < tc ? i : a.Nint, rate, sh);
     }
 }
 

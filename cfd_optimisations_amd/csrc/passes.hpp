@@ -29,9 +29,9 @@ struct PassArgs {
     float* FS32 = nullptr;       // face states, FP32-mixed: [MAXDEG][NcPad] float4
     const int* inst_off = nullptr;      // [ninst+1] interior ranges
     const int* chunk_inst = nullptr;    // [ceil(Nint/64)] instance of cell 64c
-    double* pmin_first = nullptr;       // [ntiles] per-tile CFL partial of the tile's first instance
+    double* pmin_first = nullptr;       // [ntiles] per-tile CFL partial (maximum rate) of the tile's first instance
     double* pmin_last = nullptr;        // [ntiles] ... of its last instance
-    unsigned long long* dtacc = nullptr;// [ninst] CFL minimum (bit pattern) of cells in neither
+    unsigned long long* dtacc = nullptr;// [ninst] maximum rate (bit pattern) of cells in neither
     const double* dt = nullptr;         // [ninst] dt of this step
     const BCEntry* bctab = nullptr;
     const int* ghost_bc = nullptr;
@@ -66,7 +66,7 @@ struct PassVariant {
 // Launch the passes with `grid` x `block` threads on `st`.
 hipError_t launch_pass_a(const PassVariant& v, const PassArgs& a, int grid, int block, hipStream_t st);
 hipError_t launch_pass_b(const PassVariant& v, const PassArgs& a, int grid, int block, hipStream_t st);
-// Per-instance CFL reduction: dt[b] = cfl * min over the instance's tile partials.
+// Per-instance CFL reduction: dt[b] = cfl * (1 / maximum rate over the instance's tile partials).
 hipError_t launch_dt_reduce(const PassArgs& a, double cfl, double* dt, hipStream_t st);
 // Single-kernel step over halo tiles of 256 cells, block 256, `lds` bytes of dynamic LDS.
 hipError_t launch_fused_step(const PassVariant& v, const PassArgs& a, int grid, size_t lds, hipStream_t st);

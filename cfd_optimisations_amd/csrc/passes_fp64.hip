@@ -49,12 +49,13 @@ hipError_t launch_pass_b(const PassVariant& v, const PassArgs& a, int grid, int 
     });
 }
 
-// Per-instance CFL reduction over the tile partials of pass A (block b <-> instance b).
+// Per-instance CFL reduction over the tile partials (maximum rates, see cfl_rate;
+// block b <-> instance b): dt = cfl * (1 / max rate), the value cfl * min_i(1/rate_i).
 __global__ __launch_bounds__(1024) void k_dt_reduce(PassArgs a, double cfl, double* __restrict__ dt) {
     __shared__ double sh[1024 / WAVE];
     const int b = blockIdx.x;
     const int c0 = a.inst_off[b], c1 = a.inst_off[b + 1];
-    double m = INFINITY;
+    double m = 0.0;
     if (c1 > c0) {
         // tiles of tile_cells cells, or the variable tiles of tile_start (one
         // instance: every tile; several: thread 0 searches and shares the range)
@@ -84,33 +85,33 @@ __global__ __launch_bounds__(1024) void k_dt_reduce(PassArgs a, double cfl, doub
         }
         if (a.ninst == 1) {
             // one instance: every partial is its tile's first; four loads in flight per lane
-            double m1 = INFINITY, m2 = INFINITY, m3 = INFINITY;
+            double m1 = 0.0, m2 = 0.0, m3 = 0.0;
             int tile = t0 + (int)threadIdx.x;
             const int s = blockDim.x;
             for (; tile + 3 * s <= t1; tile += 4 * s) {
-                m = fmin(m, a.pmin_first[tile]);
-                m1 = fmin(m1, a.pmin_first[tile + s]);
-                m2 = fmin(m2, a.pmin_first[tile + 2 * s]);
-                m3 = fmin(m3, a.pmin_first[tile + 3 * s]);
+                m = fmax(m, a.pmin_first[tile]);
+                m1 = fmax(m1, a.pmin_first[tile + s]);
+                m2 = fmax(m2, a.pmin_first[tile + 2 * s]);
+                m3 = fmax(m3, a.pmin_first[tile + 3 * s]);
             }
-            for (; tile <= t1; tile += s) m = fmin(m, a.pmin_first[tile]);
-            m = fmin(fmin(m, m1), fmin(m2, m3));
+            for (; tile <= t1; tile += s) m = fmax(m, a.pmin_first[tile]);
+            m = fmax(fmax(m, m1), fmax(m2, m3));
         } else {
             for (int tile = t0 + (int)threadIdx.x; tile <= t1; tile += blockDim.x) {
                 const int f0 = a.tile_start ? a.tile_start[tile] : tile * a.tile_cells;
                 const int l0 = a.tile_start ? a.tile_start[tile + 1] - 1 : min(f0 + a.tile_cells, a.Nint) - 1;
                 const int bf = instance_from(a.inst_off, a.chunk_inst, f0);
                 const int bl = instance_from(a.inst_off, a.chunk_inst, l0);
-                if (bf == b) m = fmin(m, a.pmin_first[tile]);
-                else if (bl == b) m = fmin(m, a.pmin_last[tile]);
+                if (bf == b) m = fmax(m, a.pmin_first[tile]);
+                else if (bl == b) m = fmax(m, a.pmin_last[tile]);
             }
         }
     }
-    m = block_min(m, sh);
+    m = block_max(m, sh);
     if (threadIdx.x == 0) {
-        m = fmin(m, __longlong_as_double((long long)a.dtacc[b]));
-        dt[b] = cfl * m;
-        a.dtacc[b] = dbits(INFINITY);
+        m = fmax(m, __longlong_as_double((long long)a.dtacc[b]));
+        dt[b] = cfl * (1.0 / m);
+        a.dtacc[b] = 0ull;
     }
 }
 

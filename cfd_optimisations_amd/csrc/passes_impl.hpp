@@ -73,27 +73,27 @@ __device__ __forceinline__ void tile_walk(const PassArgs& a, int& t0, int& step,
 
 __device__ __forceinline__ unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
 
-// CFL partials of one tile of cells [c0, c1]: the tile's first and last
-// instance get block minima; cells of any instance strictly inside the tile use
-// atomics.  Lanes without a cell pass i >= Nint.
+// CFL partials (maximum rate, see cfl_rate) of one tile of cells [c0, c1]: the
+// tile's first and last instance get block maxima; cells of any instance
+// strictly inside the tile use atomics.  Lanes without a cell pass i >= Nint.
 template <bool MULTI>
-__device__ __forceinline__ void cfl_tile_range(const PassArgs& a, int tile, int c0, int c1, int i, double dti,
+__device__ __forceinline__ void cfl_tile_range(const PassArgs& a, int tile, int c0, int c1, int i, double rate,
                                                double* sh) {
     if constexpr (!MULTI) {
-        const double m = block_min(dti, sh);
+        const double m = block_max(rate, sh);
         if (threadIdx.x == 0) a.pmin_first[tile] = m;
     } else {
         const int bf = instance_from(a.inst_off, a.chunk_inst, c0);
         const int bl = instance_from(a.inst_off, a.chunk_inst, c1);
-        double vf = INFINITY, vl = INFINITY;
+        double vf = 0.0, vl = 0.0;
         if (i < a.Nint) {
             const int bi = instance_from(a.inst_off, a.chunk_inst, i);
-            if (bi == bf) vf = dti;
-            else if (bi == bl) vl = dti;
-            else atomicMin(&a.dtacc[bi], dbits(dti));
+            if (bi == bf) vf = rate;
+            else if (bi == bl) vl = rate;
+            else atomicMax(&a.dtacc[bi], dbits(rate));
         }
-        const double mf = block_min(vf, sh);
-        const double ml = block_min(vl, sh);
+        const double mf = block_max(vf, sh);
+        const double ml = block_max(vl, sh);
         if (threadIdx.x == 0) {
             a.pmin_first[tile] = mf;
             a.pmin_last[tile] = ml;
@@ -103,9 +103,9 @@ __device__ __forceinline__ void cfl_tile_range(const PassArgs& a, int tile, int 
 
 // Tiles of tile_cells consecutive cells (the two-pass kernels).
 template <bool MULTI>
-__device__ __forceinline__ void cfl_tile(const PassArgs& a, int tile, int i, double dti, double* sh) {
+__device__ __forceinline__ void cfl_tile(const PassArgs& a, int tile, int i, double rate, double* sh) {
     const int c0 = tile * a.tile_cells;
-    cfl_tile_range<MULTI>(a, tile, c0, min(c0 + a.tile_cells, a.Nint) - 1, i, dti, sh);
+    cfl_tile_range<MULTI>(a, tile, c0, min(c0 + a.tile_cells, a.Nint) - 1, i, rate, sh);
 }
 
 // Face-state planes: FP64 slot k of cell i is two double2 at planes 2k, 2k+1;
@@ -281,7 +281,7 @@ __device__ __forceinline__ double pass_a_cell(const PassArgs& a, int i, const do
             }
         }
     }
-    if constexpr (CFL) return 1.0 / ((double)iv * lam);
+    if constexpr (CFL) return cfl_rate((double)iv * lam);
     return 0.0;
 }
 
@@ -360,7 +360,7 @@ __device__ __forceinline__ double pass_a_cell_quad(const PassArgs& a, int i, int
             }
         }
     }
-    if constexpr (CFL) return v == 0 ? 1.0 / ((double)iv * lam) : INFINITY;
+    if constexpr (CFL) return v == 0 ? cfl_rate((double)iv * lam) : 0.0;
     return 0.0;
 }
 
@@ -385,12 +385,12 @@ __global__ __launch_bounds__(256) HCFD_WAVES_A_ATTR void k_pass_a(PassArgs a) {
             }
             __syncthreads();
         }
-        double dti = INFINITY;
+        double rate = 0.0;
         if (i < a.Nint) {
-            if constexpr (LPC == 4) dti = pass_a_cell_quad<T, W, MAXDEG, CFL>(a, i, threadIdx.x & 3);
-            else dti = pass_a_cell<T, W, MAXDEG, CFL>(a, i, sU, tb, min(a.tile_cells, a.Nc - tb));
+            if constexpr (LPC == 4) rate = pass_a_cell_quad<T, W, MAXDEG, CFL>(a, i, threadIdx.x & 3);
+            else rate = pass_a_cell<T, W, MAXDEG, CFL>(a, i, sU, tb, min(a.tile_cells, a.Nc - tb));
         }
-        if constexpr (CFL) cfl_tile<MULTI>(a, tile, i, dti, sh);
+        if constexpr (CFL) cfl_tile<MULTI>(a, tile, i, rate, sh);
     }
 }
 
