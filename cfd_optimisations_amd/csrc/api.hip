@@ -1195,9 +1195,12 @@ int hcfd_autotune(hcfd_ctx* c, int steps, double cfl, const char* cache_path, hc
         if (!c) fail(HCFD_ERR_ARG, "null");
         require_bcs(c);
         if (steps <= 0) steps = 10;
-        char key[160];
-        snprintf(key, sizeof key, "gfx950 Nc=%d Nf=%d Nint=%d ninst=%d prec=%d flags=%d deg=%d", c->Nc, c->Nf, c->Nint,
-                 c->ninst, c->precision, c->flags & HCFD_FLAG_EXACT, c->deg_bucket);
+        // the key holds what decides the candidates' speed: sizes, precision, cell
+        // order (flags other than graph replay), slot bucket, integrator stages
+        char key[200];
+        snprintf(key, sizeof key, "gfx950 Nc=%d Nf=%d Nint=%d ninst=%d prec=%d flags=%d deg=%d stages=%d fused_ok=%d",
+                 c->Nc, c->Nf, c->Nint, c->ninst, c->precision, c->flags & ~HCFD_FLAG_GRAPH, c->deg_bucket, c->stages,
+                 c->fused_ok ? 1 : 0);
         // cache lookup
         if (cache_path) {
             std::ifstream in(cache_path);
@@ -1208,7 +1211,11 @@ int hcfd_autotune(hcfd_ctx* c, int steps, double cfl, const char* cache_path, hc
                 hcfd_schedule s{};
                 if (sscanf(line.c_str() + bar + 1, "%d %d %d %d %d %d", &s.block, &s.lanes_per_cell,
                            &s.aosoa_width, &s.persistent, &s.xcd_remap, &s.fused) == 6) {
-                    check_schedule(c, s);
+                    try {
+                        check_schedule(c, s);
+                    } catch (const Error&) {
+                        continue;  // an entry this context cannot run: tune instead
+                    }
                     change_width(c, s.aosoa_width);
                     c->sched = s;
                     drop_graph(c);

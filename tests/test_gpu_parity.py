@@ -442,3 +442,40 @@ def test_free_stream_preserved_on_16m_ramp(prec):
     for v in range(4):
         scale = np.abs(U0[v]).max()
         assert np.abs(U[v] - U0[v]).max() <= tol * max(scale, 1.0), v
+
+
+def test_autotune_restores_state_and_uses_cache(tmp_path):
+    """The autotuner times candidate schedules on the live state and puts the
+    state, clock and last dt back; the choice is cached by mesh size, precision
+    and cell order; a cached line this context cannot run is ignored; stepping
+    afterwards is bit-identical to the oracle whatever schedule won."""
+    case = C.riemann_case(60, 40, seed=8)
+    od, U0 = oracle_for(case)
+    U5, t5 = od.run(U0, 5, cfl=0.5)
+    U10, t10 = od.run(U5, 5, cfl=0.5)
+    cache = str(tmp_path / "tune.txt")
+    with C.make_context(case, L.FP64, L.FLAG_ORDER_SFC) as ctx:
+        ctx.step(5, cfl=0.5)
+        dt5 = ctx.last_dt()
+        best = ctx.autotune(steps=2, cfl=0.5, cache=cache)
+        assert ctx.schedule() == best
+        assert np.array_equal(ctx.get_state(), U5)
+        assert ctx.sync() == t5 and ctx.last_dt() == dt5
+        t = ctx.step(5, cfl=0.5)
+        assert abs(t - (t5 + t10)) <= 1e-15 * t     # the oracle's t restarts at 0 per call
+        assert np.array_equal(ctx.get_state(), U10)
+    lines = open(cache).read().splitlines()
+    assert len(lines) == 1 and "|" in lines[0]
+    key = lines[0].split("|")[0]
+    # a second context of the same mesh takes the cached choice without tuning
+    with C.make_context(case, L.FP64, L.FLAG_ORDER_SFC) as ctx:
+        assert ctx.autotune(steps=2, cfl=0.5, cache=cache) == best
+    assert len(open(cache).read().splitlines()) == 1
+    # an unusable cached line (single kernel with 64-thread blocks) is skipped
+    open(cache, "w").write(key + "|64 1 1 0 1 1\n")
+    with C.make_context(case, L.FP64, L.FLAG_ORDER_SFC) as ctx:
+        best2 = ctx.autotune(steps=2, cfl=0.5, cache=cache)
+        assert not (best2["fused"] and best2["block"] != 256)
+        ctx.step(5, cfl=0.5)
+        assert np.array_equal(ctx.get_state(), U5)
+    assert len(open(cache).read().splitlines()) == 2
