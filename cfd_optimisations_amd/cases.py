@@ -158,8 +158,10 @@ def wedge_batch(n: int, seed: int, mach=(1.5, 4.0), angle=(5.0, 25.0), tris=(20e
 # Workloads as quoted for this residual (sizes in quads; each quad is 2 triangles).
 WORKLOADS = {
     "c1_wedge_cpu": dict(kind="ramp", nx=224, ny=112, mach=2.5, angle=15.0, steps=2000, cfl=0.5),
-    # CFL 0.4: at 0.5 the reference method hits a non-physical face state near the
-    # four-state junction around step 450 on this mesh; at 0.4 all 500 steps stay physical
+    # CFL 0.4: the reference method (forward Euler, not positivity-preserving)
+    # first produces a non-physical state near the four-state junction after
+    # 426-450 steps at CFL 0.5, 551-575 at 0.4, 626-650 at 0.35 and 726-750 at
+    # 0.3 on this mesh; at 0.4 the quoted 500 steps stay physical
     "c2_riemann_fp64": dict(kind="riemann", nx=2000, ny=1000, steps=500, cfl=0.4, precision=L.FP64),
     "c3_ramp_16m": dict(kind="ramp", nx=4000, ny=2000, mach=3.0, angle=20.0, steps=1000, cfl=0.5),
     "c4_batch_fp64": dict(kind="batch", n=256, seed=1, mach=(1.5, 4.0), angle=(5.0, 25.0), tris=(20e3, 500e3),

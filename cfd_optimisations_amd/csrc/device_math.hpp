@@ -32,16 +32,30 @@ __device__ __forceinline__ float quot(float a, float b) { return a * __builtin_a
 __device__ __forceinline__ double vsqrt(double x) { return sqrt(x); }
 __device__ __forceinline__ float vsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
+// a * b + c.  FP64: product and sum rounded separately, the reference's
+// arithmetic (IEEE addition commutes exactly, so x + a * b written as
+// madd(a, b, x) keeps its value).  FP32 (the mixed path): one fused
+// multiply-add, a single rounding and one instruction instead of two.
+__device__ __forceinline__ double madd(double a, double b, double c) { return a * b + c; }
+__device__ __forceinline__ float madd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+// G . r of one face: g0 r0 + g1 r1 (Limiter.cpp:262, Flux.cpp:83-87).
+template <typename T>
+__device__ __forceinline__ T dot_r(T g0, T g1, T r0, T r1) { return madd(g0, r0, g1 * r1); }
+
 // Flux::EulerFlux, src/Base/Flux.cpp:89-97, given ir = 1 / U[0] (the value
 // Rusanov already holds for the same state).
 template <typename T>
 __device__ __forceinline__ void euler_flux(T gma, const T U[NV], T ir, T n0, T n1, T F[NV]) {
     const T gmo = gma - T(1);
-    const T P = gmo * (U[3] - T(0.5) * ir * (U[1] * U[1] + U[2] * U[2]));
-    F[0] = n0 * U[1] + n1 * U[2];
-    F[1] = n0 * (U[1] * U[1] * ir + P) + n1 * U[2] * U[1] * ir;
-    F[2] = n0 * U[1] * U[2] * ir + n1 * (U[2] * U[2] * ir + P);
-    F[3] = n0 * (U[3] + P) * U[1] * ir + n1 * (U[3] + P) * U[2] * ir;
+    // P = gmo (U3 - 0.5 ir (U1 U1 + U2 U2)); each madd below is one a*b + c of
+    // the reference's expression, grouped as it is written there
+    const T P = gmo * madd(-(T(0.5) * ir), madd(U[1], U[1], U[2] * U[2]), U[3]);
+    const T E = U[3] + P;
+    F[0] = madd(n0, U[1], n1 * U[2]);
+    F[1] = madd(n0, madd(U[1] * U[1], ir, P), n1 * U[2] * U[1] * ir);
+    F[2] = madd(n1, madd(U[2] * U[2], ir, P), n0 * U[1] * U[2] * ir);
+    F[3] = madd(n0 * E * U[1], ir, n1 * E * U[2] * ir);
 }
 
 // Flux::Rusanov, src/Base/Flux.cpp:52-81.
@@ -50,10 +64,10 @@ __device__ __forceinline__ void rusanov(T gma, const T UL[NV], const T UR[NV], T
     const T gmo = gma - T(1);
     const T irl = recip(UL[0]);
     const T irr = recip(UR[0]);
-    const T nvl = (n0 * UL[1] + n1 * UL[2]) * irl;
-    const T nvr = (n0 * UR[1] + n1 * UR[2]) * irr;
-    const T qql = T(0.5) * (UL[1] * UL[1] + UL[2] * UL[2]) * irl;
-    const T qqr = T(0.5) * (UR[1] * UR[1] + UR[2] * UR[2]) * irr;
+    const T nvl = madd(n0, UL[1], n1 * UL[2]) * irl;
+    const T nvr = madd(n0, UR[1], n1 * UR[2]) * irr;
+    const T qql = T(0.5) * madd(UL[1], UL[1], UL[2] * UL[2]) * irl;
+    const T qqr = T(0.5) * madd(UR[1], UR[1], UR[2] * UR[2]) * irr;
     const T pl = gmo * (UL[3] - qql);
     const T pr = gmo * (UR[3] - qqr);
     const T al = vsqrt(gma * pl * irl);
@@ -65,7 +79,7 @@ __device__ __forceinline__ void rusanov(T gma, const T UL[NV], const T UR[NV], T
     euler_flux(gma, UL, irl, n0, n1, FL);
     euler_flux(gma, UR, irr, n0, n1, FR);
 #pragma unroll
-    for (int v = 0; v < NV; v++) F[v] = T(0.5) * ((FL[v] + FR[v]) - S * (UR[v] - UL[v]));
+    for (int v = 0; v < NV; v++) F[v] = T(0.5) * madd(-S, UR[v] - UL[v], FL[v] + FR[v]);
 }
 
 // Limiter face candidate, src/Base/Limiter.cpp:262-266 (eps = 1e-9, :20):

@@ -162,10 +162,10 @@ __device__ __forceinline__ void fused_cell_a(const PassArgs& a, const TileLists&
         const T wo = isR ? omI : If, wn = isR ? If : omI;
 #pragma unroll
         for (int v = 0; v < NV; v++) {
-            const T uf = u[v] * wo + un[v] * wn;
+            const T uf = madd(u[v], wo, un[v] * wn);
             const T au = uf * c.A[k];
-            g2[v][0] = g2[v][0] + au * c.n0[k];
-            g2[v][1] = g2[v][1] + au * c.n1[k];
+            g2[v][0] = madd(au, c.n0[k], g2[v][0]);
+            g2[v][1] = madd(au, c.n1[k], g2[v][1]);
             umin[v] = dmin(umin[v], un[v]);
             umax[v] = dmax(umax[v], un[v]);
         }
@@ -183,7 +183,7 @@ __device__ __forceinline__ void fused_cell_a(const PassArgs& a, const TileLists&
         const T dmn = umin[v] - u[v], dmx = umax[v] - u[v];
 #pragma unroll
         for (int k = 0; k < MAXDEG; k++)
-            if (k < deg) phi[v] = dmin(phi[v], limiter_face(g2[v][0] * c.r0[k] + g2[v][1] * c.r1[k], dmn, dmx));
+            if (k < deg) phi[v] = dmin(phi[v], limiter_face(dot_r(g2[v][0], g2[v][1], c.r0[k], c.r1[k]), dmn, dmx));
     }
 #pragma unroll
     for (int k = 0; k < MAXDEG; k++) {
@@ -191,8 +191,8 @@ __device__ __forceinline__ void fused_cell_a(const PassArgs& a, const TileLists&
             T d[NV], uf[NV];
 #pragma unroll
             for (int v = 0; v < NV; v++) {
-                d[v] = g2[v][0] * c.r0[k] + g2[v][1] * c.r1[k];
-                uf[v] = u[v] + phi[v] * d[v];
+                d[v] = dot_r(g2[v][0], g2[v][1], c.r0[k], c.r1[k]);
+                uf[v] = madd(phi[v], d[v], u[v]);
             }
             FT::put(sFS + (size_t)(k * FT::PER) * fs_pitch, fs_pitch, l, uf);
             if (own && ((ghosts >> k) & 1u)) {
@@ -205,7 +205,7 @@ __device__ __forceinline__ void fused_cell_a(const PassArgs& a, const TileLists&
                 T ug[NV], ufg[NV];
                 FT::get(sU, su_pitch, ln[k], ug);
 #pragma unroll
-                for (int v = 0; v < NV; v++) ufg[v] = ug[v] + phi[v] * d[v];
+                for (int v = 0; v < NV; v++) ufg[v] = madd(phi[v], d[v], ug[v]);
                 FT::put(sFS, fs_pitch, ln[k], ufg);
             }
         }

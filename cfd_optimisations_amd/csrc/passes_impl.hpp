@@ -229,10 +229,10 @@ __device__ __forceinline__ double pass_a_cell(const PassArgs& a, int i, const do
 #pragma unroll
         for (int v = 0; v < NV; v++) {
             const T un = (T)und[v];
-            const T uf = u[v] * wo + un * wn;
+            const T uf = madd(u[v], wo, un * wn);
             const T au = uf * A;
-            g[v][0] = g[v][0] + au * n0;
-            g[v][1] = g[v][1] + au * n1;
+            g[v][0] = madd(au, n0, g[v][0]);
+            g[v][1] = madd(au, n1, g[v][1]);
             umin[v] = dmin(umin[v], un);
             umax[v] = dmax(umax[v], un);
         }
@@ -252,7 +252,7 @@ __device__ __forceinline__ double pass_a_cell(const PassArgs& a, int i, const do
 #pragma unroll
         for (int k = 0; k < MAXDEG; k++) {
             if (k < deg) {
-                const T d = g[v][0] * r0[k] + g[v][1] * r1[k];
+                const T d = dot_r(g[v][0], g[v][1], r0[k], r1[k]);
                 phi[v] = dmin(phi[v], limiter_face(d, dmn, dmx));
             }
         }
@@ -265,8 +265,8 @@ __device__ __forceinline__ double pass_a_cell(const PassArgs& a, int i, const do
             T d[NV], uf[NV];
 #pragma unroll
             for (int v = 0; v < NV; v++) {
-                d[v] = g[v][0] * r0[k] + g[v][1] * r1[k];
-                uf[v] = u[v] + phi[v] * d[v];
+                d[v] = dot_r(g[v][0], g[v][1], r0[k], r1[k]);
+                uf[v] = madd(phi[v], d[v], u[v]);
             }
             store_fs(FS, P, k, i, uf);
             if ((ghosts >> k) & 1u) {
@@ -276,7 +276,7 @@ __device__ __forceinline__ double pass_a_cell(const PassArgs& a, int i, const do
                 tile_or_global_cell<W>(a, sU, tb, tc, gc, ugd);
                 T ufg[NV];
 #pragma unroll
-                for (int v = 0; v < NV; v++) ufg[v] = (T)ugd[v] + phi[v] * d[v];
+                for (int v = 0; v < NV; v++) ufg[v] = madd(phi[v], d[v], (T)ugd[v]);
                 store_fs(FS, P, 0, gc, ufg);
             }
         }
@@ -331,10 +331,10 @@ __device__ __forceinline__ double pass_a_cell_quad(const PassArgs& a, int i, int
         const T un = (T)a.U[cidx<W, NV>(nb, v)];
         const T omI = T(1) - If;
         const bool isR = (side >> k) & 1;
-        const T uf = u * (isR ? omI : If) + un * (isR ? If : omI);
+        const T uf = madd(u, isR ? omI : If, un * (isR ? If : omI));
         const T au = uf * A;
-        g0 = g0 + au * n0;
-        g1 = g1 + au * n1;
+        g0 = madd(au, n0, g0);
+        g1 = madd(au, n1, g1);
         umin = dmin(umin, un);
         umax = dmax(umax, un);
         if (nb >= a.Nint) ghosts |= 1u << k;
@@ -346,17 +346,17 @@ __device__ __forceinline__ double pass_a_cell_quad(const PassArgs& a, int i, int
     const T dmn = umin - u, dmx = umax - u;
 #pragma unroll
     for (int k = 0; k < MAXDEG; k++)
-        if (k < deg) phi = dmin(phi, limiter_face(g0 * r0[k] + g1 * r1[k], dmn, dmx));
+        if (k < deg) phi = dmin(phi, limiter_face(dot_r(g0, g1, r0[k], r1[k]), dmn, dmx));
     T* FS = PT::fs(a);
 #pragma unroll
     for (int k = 0; k < MAXDEG; k++) {
         if (k < deg) {
-            const T d = g0 * r0[k] + g1 * r1[k];
-            FS[fs_index<T>(P, k, i, v)] = u + phi * d;
+            const T d = dot_r(g0, g1, r0[k], r1[k]);
+            FS[fs_index<T>(P, k, i, v)] = madd(phi, d, u);
             if ((ghosts >> k) & 1u) {
                 const int gc = a.cnb[k * P + i] & SLOT_NB_MASK;
                 const T ug = (T)a.U[cidx<W, NV>(gc, v)];
-                FS[fs_index<T>(P, 0, gc, v)] = ug + phi * d;
+                FS[fs_index<T>(P, 0, gc, v)] = madd(phi, d, ug);
             }
         }
     }

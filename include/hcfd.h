@@ -249,8 +249,11 @@ int hcfd_copy_bandwidth(long long bytes, int reps, double* gbs);
 int hcfd_set_schedule(hcfd_ctx* ctx, const hcfd_schedule* s);
 int hcfd_get_schedule(hcfd_ctx* ctx, hcfd_schedule* s);
 /* Project autotuner: times candidate schedules for `steps` steps each on the
- * current state (state restored afterwards), keeps the fastest and, when
- * cache_path is non-NULL, records it there keyed by mesh size and precision. */
+ * current state (state, clock and last dt restored afterwards), keeps the
+ * fastest and, when cache_path is non-NULL, records it there keyed by mesh
+ * size, batch size, precision, flags, cell degree, integrator stages and
+ * whether the single kernel fits.  A cached entry is used without timing; one
+ * this context cannot run is ignored and the context is tuned afresh. */
 int hcfd_autotune(hcfd_ctx* ctx, int steps, double cfl, const char* cache_path, hcfd_schedule* best);
 
 /* ---- batches of independent instances (one launch per pass for all) ------- */
