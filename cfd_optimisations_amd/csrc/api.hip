@@ -203,7 +203,10 @@ hcfd_ctx* create_ctx(int n, const hcfd_mesh_view* views, int precision, int flag
         Nn += views[k].Nn;
         ncn += views[k].cn_off[views[k].Nc];
     }
-    if (Nc >= (1LL << 30) || Nf >= (1LL << 30) || ncn >= (1LL << 31)) fail(HCFD_ERR_ARG, "mesh too large");
+    // neighbour indices are packed in 29 bits, and per-cell planes (padded to
+    // PAD cells, at most 8 bytes an entry) are addressed with 32-bit byte offsets
+    if (Nc > (1LL << 29) - PAD || Nf >= (1LL << 30) || ncn >= (1LL << 31))
+        fail(HCFD_ERR_ARG, "mesh too large: at most 2^29 - 64 cells and 2^30 - 1 faces");
     std::vector<int> fc(2 * Nf), fn(2 * Nf), cn_off(Nc + 1), cn_con(ncn), bcid(Nf, -1);
     std::vector<double> nodes(2 * Nn);
     std::vector<unsigned char> ghost(Nc, 0);
@@ -337,7 +340,8 @@ hcfd_ctx* create_ctx(int n, const hcfd_mesh_view* views, int precision, int flag
         int lim = 0;
         HIPCHK(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
         // runnable when the largest tile's halo fits in LDS with two blocks per CU
-        c->fused_ok = c->fused_lds <= (size_t)std::min(lim, 80 * 1024);
+        c->fused_ok = c->fused_lds <= (size_t)std::min(lim, 80 * 1024) &&
+                      (precision != HCFD_MIXED || (c->halo.lt && c->halo.unresolved == 0));
         // default only for compact tiles (at most 5% of the tiles had to be split):
         // strip-shaped tiles, as a reverse Cuthill-McKee order makes, recompute more
         // halo than they update; Hilbert tiles keep the halo near a quarter
@@ -458,6 +462,9 @@ Launch make_launch(hcfd_ctx* c, bool cfl, int cur, int pass) {
     a.s_off = c->halo.s_off;
     a.s_ngt = c->halo.s_ngt;
     a.s2 = c->halo.s2;
+    a.lt = c->halo.lt;
+    a.lh = c->halo.lh;
+    a.nh1tot = std::max(c->halo.nh1, 1LL);
     a.smax = c->halo.smax;
     a.fsmax = c->halo.fsmax;
     a.h1max = c->halo.h1max;

@@ -95,6 +95,15 @@ __device__ __forceinline__ T limiter_face(T d, T umin_minus_u, T umax_minus_u) {
     return take ? dmin(T(1), q) : T(1);
 }
 
+// Element i of a per-cell plane: the plane's base is uniform across the wave
+// and i enters as a 32-bit byte offset, so each load is one scalar base plus
+// the lane's offset (global_load's saddr form) with no 64-bit vector add per
+// plane.  Needs i * sizeof(T) < 2^32: contexts hold at most 2^29 cells.
+template <typename T>
+__device__ __forceinline__ T plane_at(const T* __restrict__ base, int i) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (unsigned)i * (unsigned)sizeof(T));
+}
+
 // Cell-state access with an AoSoA tile of width W: component c of cell i of an
 // array with NC components lives at (i/W)*NC*W + c*W + i%W.  W == 1 is AoS,
 // W >= the cell count behaves as SoA.

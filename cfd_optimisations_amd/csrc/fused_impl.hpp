@@ -31,38 +31,6 @@ namespace hcfd {
 
 constexpr int FUSED_TILE = 256;
 
-// Local index of cell j among the tile's staged cells, or -1.
-struct TileLists {
-    int tb, tc, nh1, ngt, ns2, Nint;
-    const int* sH1;
-    const int* sS2;
-};
-
-__device__ __forceinline__ int lds_lower_bound(const int* a, int n, int x) {
-    int lo = 0, hi = n;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (a[mid] < x) lo = mid + 1; else hi = mid;
-    }
-    return lo;
-}
-
-__device__ __forceinline__ int local_of(const TileLists& L, int j) {
-    const unsigned d = (unsigned)(j - L.tb);
-    if (d < (unsigned)L.tc) return (int)d;
-    if (j < L.Nint) {
-        const int p = lds_lower_bound(L.sH1, L.nh1, j);
-        if (p < L.nh1 && L.sH1[p] == j) return L.tc + p;
-    } else {
-        const int p = lds_lower_bound(L.sS2, L.ngt, j);
-        if (p < L.ngt && L.sS2[p] == j) return L.tc + L.nh1 + p;
-    }
-    const int nr = L.ns2 - L.ngt;
-    const int p = lds_lower_bound(L.sS2 + L.ngt, nr, j);
-    if (p < nr && L.sS2[L.ngt + p] == j) return L.tc + L.nh1 + L.ngt + p;
-    return -1;
-}
-
 // LDS element types: FP64 cell state / face state = two double2; FP32 = one float4.
 template <typename T>
 struct FusedTypes;
@@ -94,6 +62,7 @@ struct FusedTypes<float> {
 template <typename T, int MAXDEG>
 struct CellIn {
     int nbx[MAXDEG];  // neighbour | twin << 29, or SLOT_EMPTY
+    int lq[MAXDEG];   // the neighbour's local index in this tile's staging (-1: empty slot)
     T A[MAXDEG], n0[MAXDEG], n1[MAXDEG], If[MAXDEG], r0[MAXDEG], r1[MAXDEG];
     T iv;
     int side;
@@ -104,10 +73,10 @@ __device__ __forceinline__ void load_cell_in(const PassArgs& a, int g, CellIn<T,
     const long long P = a.NcPad;
     const T* __restrict__ gp = PassATypes<T>::geo(a);
     c.iv = PassATypes<T>::iv(a, g);
-    c.side = a.cside[g];
+    c.side = plane_at(a.cside, g);
 #pragma unroll
     for (int k = 0; k < MAXDEG; k++) {
-        c.nbx[k] = a.cnb[k * P + g];
+        c.nbx[k] = cnb_at(a, k, g);
         c.A[k] = geo<MAXDEG>(gp, P, GEO_A, k, g);
         c.n0[k] = geo<MAXDEG>(gp, P, GEO_N0, k, g);
         c.n1[k] = geo<MAXDEG>(gp, P, GEO_N1, k, g);
@@ -148,7 +117,11 @@ __device__ __forceinline__ void fused_cell_a(const PassArgs& a, const TileLists&
         if (nbx == SLOT_EMPTY) continue;
         deg = k + 1;
         const int nb = nbx & SLOT_NB_MASK;
-        int q = local_of(L, nb);
+        // FP64: a search of the tile's lists in LDS (its registers are spoken
+        // for); FP32: the index read with the cell's inputs
+        int q;
+        if constexpr (sizeof(T) == 8) q = local_of(L, nb);
+        else q = c.lq[k];
         if (q < 0) {
             bad = true;
             q = l;
@@ -232,6 +205,16 @@ __device__ __forceinline__ double cfl_bound_in(double gamma, const CellIn<T, MAX
     return cfl_rate((double)ci.iv * lam);
 }
 
+// The neighbours' local indices of a staged cell: entry e of the planes of
+// `loc` (pitch lpitch), the cell's own entry of lt for a tile cell, its H1
+// entry of lh for a ring cell.
+template <typename T, int MAXDEG>
+__device__ __forceinline__ void load_local(const short* __restrict__ loc, long long lpitch, int e,
+                                           CellIn<T, MAXDEG>& c) {
+#pragma unroll
+    for (int k = 0; k < MAXDEG; k++) c.lq[k] = plane_at(loc + k * lpitch, e);
+}
+
 template <typename T, int MAXDEG>
 __device__ __forceinline__ double cfl_bound(const PassArgs& a, int g, const double u[NV]) {
     CellIn<T, MAXDEG> ci;
@@ -305,9 +288,12 @@ k_fused_step(
     const int su_pitch = a.smax, fs_pitch = a.fsmax;
     V* sU = reinterpret_cast<V*>(fused_lds);
     V* sFS = sU + (size_t)FT::PER * su_pitch;
-    int* sH1 = reinterpret_cast<int*>(sFS + (size_t)FT::PER * MAXDEG * fs_pitch);
+    TileMin* tmin = reinterpret_cast<TileMin*>(sFS + (size_t)FT::PER * MAXDEG * fs_pitch);
+    // FP64: the tile's H1 and S2 id lists, searched for neighbours' local
+    // indices, after the 32-byte CFL slots
+    constexpr bool search = sizeof(T) == 8;
+    int* sH1 = reinterpret_cast<int*>(reinterpret_cast<char*>(tmin) + 32);
     int* sS2 = sH1 + ((a.h1max + 3) & ~3);
-    TileMin* tmin = reinterpret_cast<TileMin*>(sS2 + ((a.s2max + 3) & ~3));
     const int tid = threadIdx.x;
     if (tid == 0) {
         tmin->first = 0ull;
@@ -340,8 +326,10 @@ k_fused_step(
         // this lane's tile cell: its inputs are in flight while the tile is staged
         CellIn<T, MAXDEG> ci;
         double uown[NV] = {0, 0, 0, 0};  // U^n of the lane's tile cell (FP64: read back from LDS in pass B)
+        // (FP32: with its neighbours' local indices)
         if (hasT) {
             load_cell_in<T, MAXDEG>(a, i, ci);
+            if constexpr (!search) load_local(a.lt, a.NcPad, i, ci);
             load_cell<W, NV>(a.U, i, uown);
         }
         // the halo ring goes to one wavefront, rotated from tile to tile so the
@@ -352,7 +340,10 @@ k_fused_step(
         const int rtid = (tid + WAVE * ((blockIdx.x + tile) & (FUSED_TILE / WAVE - 1))) & (FUSED_TILE - 1);
         const int qh = FUSED_TILE - 1 - rtid;
         CellIn<T, MAXDEG> ch;
-        if (early_ring && qh < L.nh1) load_cell_in<T, MAXDEG>(a, a.h1[h0 + qh], ch);
+        if (early_ring && qh < L.nh1) {
+            load_cell_in<T, MAXDEG>(a, a.h1[h0 + qh], ch);
+            if constexpr (!search) load_local(a.lh, a.nh1tot, h0 + qh, ch);
+        }
         // staged halo cell q = tid (ring H1, then S2): its id and state are in
         // flight before the barrier too; further rounds (more than 256 staged
         // halo cells) load after it
@@ -380,8 +371,10 @@ k_fused_step(
                 g = q < L.nh1 ? a.h1[h0 + q] : a.s2[s0 + q - L.nh1];
                 load_cell<W, NV>(a.U, g, ud);
             }
-            if (q < L.nh1) sH1[q] = g;
-            else sS2[q - L.nh1] = g;
+            if constexpr (search) {
+                if (q < L.nh1) sH1[q] = g;
+                else sS2[q - L.nh1] = g;
+            }
             T ut[NV];
 #pragma unroll
             for (int v = 0; v < NV; v++) ut[v] = (T)ud[v];
@@ -395,7 +388,10 @@ k_fused_step(
         for (int k = 0; k < MAXDEG; k++) lnb[k] = -1;
         if (hasT) fused_cell_a<T, MAXDEG>(a, L, ci, tid, true, sU, su_pitch, sFS, fs_pitch, lnb, bad);
         for (int q = qh; q < L.nh1; q += FUSED_TILE) {
-            if (!early_ring || q != qh) load_cell_in<T, MAXDEG>(a, a.h1[h0 + q], ch);
+            if (!early_ring || q != qh) {
+                load_cell_in<T, MAXDEG>(a, a.h1[h0 + q], ch);
+                if constexpr (!search) load_local(a.lh, a.nh1tot, h0 + q, ch);
+            }
             int ln[MAXDEG];
             fused_cell_a<T, MAXDEG>(a, L, ch, L.tc + q, false, sU, su_pitch, sFS, fs_pitch, ln, bad);
         }

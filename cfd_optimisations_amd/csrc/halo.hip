@@ -213,6 +213,70 @@ void build_lists(int Nint, long long P, int nslot, const int* cnb, const std::ve
     HIPCHK(hipStreamSynchronize(st));
 }
 
+// Local indices of the neighbours of tile t's cells (T, then its H1 entries):
+// one block per tile, the tile's H1 and S2 lists staged in LDS and searched.
+__global__ void k_local_nb(int Nint, long long P, int nslot, const int* __restrict__ cnb,
+                           const int* __restrict__ tile_start, const int* __restrict__ h1_off,
+                           const int* __restrict__ h1, const int* __restrict__ s_off, const int* __restrict__ s_ngt,
+                           const int* __restrict__ s2, long long nh1tot, short* __restrict__ lt,
+                           short* __restrict__ lh, unsigned long long* __restrict__ unresolved) {
+    extern __shared__ int lists[];
+    const int t = blockIdx.x;
+    TileLists L;
+    L.tb = tile_start[t];
+    L.tc = tile_start[t + 1] - L.tb;
+    const int h0 = h1_off[t], s0 = s_off[t];
+    L.nh1 = h1_off[t + 1] - h0;
+    L.ns2 = s_off[t + 1] - s0;
+    L.ngt = s_ngt[t];
+    L.Nint = Nint;
+    int* sH1 = lists;
+    int* sS2 = lists + L.nh1;
+    for (int q = threadIdx.x; q < L.nh1; q += blockDim.x) sH1[q] = h1[h0 + q];
+    for (int q = threadIdx.x; q < L.ns2; q += blockDim.x) sS2[q] = s2[s0 + q];
+    L.sH1 = sH1;
+    L.sS2 = sS2;
+    __syncthreads();
+    unsigned long long miss = 0;
+    for (int q = threadIdx.x; q < L.tc + L.nh1; q += blockDim.x) {
+        const int c = q < L.tc ? L.tb + q : sH1[q - L.tc];
+        for (int k = 0; k < nslot; k++) {
+            const int x = cnb[k * P + c];
+            int l = -1;
+            if (x != SLOT_EMPTY) {
+                l = local_of(L, x & SLOT_NB_MASK);
+                if (l < 0) miss++;
+            }
+            if (q < L.tc) lt[k * P + c] = (short)l;
+            else lh[k * nh1tot + h0 + (q - L.tc)] = (short)l;
+        }
+    }
+    if (miss) atomicAdd(unresolved, miss);
+}
+
+// The 16-bit local tables of the final tiling (local indices stay below 2^15
+// for every tiling the single kernel can run: its LDS holds at most a few
+// thousand staged cells).
+void build_local_tables(int Nint, long long P, int nslot, const int* cnb, HaloTables& out, hipStream_t st) {
+    out.unresolved = 0;
+    if (out.ntiles == 0 || out.smax >= (1 << 15)) return;
+    const size_t lds = sizeof(int) * ((size_t)out.h1max + out.s2max);
+    if (lds > 64 * 1024) return;
+    out.lt = dalloc<short>((size_t)nslot * P);
+    out.lh = dalloc<short>((size_t)nslot * std::max(out.nh1, 1LL));
+    HIPCHK(hipMemsetAsync(out.lt, 0xff, sizeof(short) * (size_t)nslot * P, st));
+    unsigned long long* d_miss = dalloc<unsigned long long>(1);
+    HIPCHK(hipMemsetAsync(d_miss, 0, sizeof(unsigned long long), st));
+    k_local_nb<<<out.ntiles, 256, lds, st>>>(Nint, P, nslot, cnb, out.tile_start, out.h1_off, out.h1, out.s_off,
+                                             out.s_ngt, out.s2, std::max(out.nh1, 1LL), out.lt, out.lh, d_miss);
+    HIPCHK(hipGetLastError());
+    unsigned long long miss = 0;
+    HIPCHK(hipMemcpyAsync(&miss, d_miss, sizeof(miss), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    (void)hipFree(d_miss);
+    out.unresolved = (long long)miss;
+}
+
 }  // namespace
 
 void build_halo_tables(int Nint, int NcPad, int nslot, const int* cnb, int TC, bool mixed, size_t lds_budget,
@@ -259,10 +323,12 @@ void build_halo_tables(int Nint, int NcPad, int nslot, const int* cnb, int TC, b
         out.TC = TC;
     }
     out.split_tiles = out.ntiles - (Nint + TC - 1) / TC;
+    if (mixed) build_local_tables(Nint, NcPad, nslot, cnb, out, st);
 }
 
 void free_halo_tables(HaloTables& t) {
-    for (void* p : {(void*)t.tile_start, (void*)t.h1_off, (void*)t.h1, (void*)t.s_off, (void*)t.s_ngt, (void*)t.s2})
+    for (void* p : {(void*)t.tile_start, (void*)t.h1_off, (void*)t.h1, (void*)t.s_off, (void*)t.s_ngt, (void*)t.s2,
+                    (void*)t.lt, (void*)t.lh})
         if (p) (void)hipFree(p);
     t = HaloTables{};
 }

@@ -9,7 +9,10 @@
 //          state), then the remaining neighbours of H1 cells (interior or ghost).
 // Each list is sorted by cell id.  A block stages the states of T, H1, S2 (in
 // that order: local index = position) and the face states of T, H1 and the
-// ghosts of T in LDS.
+// ghosts of T in LDS.  For FP32-mixed the local index of every neighbour of a
+// T or H1 cell is resolved once, at build time, into 16-bit tables (lt, lh),
+// so the step kernel reads it instead of searching the lists (in FP64 the
+// registers the table needs are not free, and the kernel searches).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -30,7 +33,44 @@ struct HaloTables {
     int fsmax = 0;          // max over tiles of |T| + |H1| + ghosts of T (staged face states)
     int h1max = 0, s2max = 0;
     long long nh1 = 0, ns2 = 0;
+    // local index (position among the tile's staged cells) of neighbour slot k:
+    short* lt = nullptr;    // [nslot][NcPad] of interior cell i in its own tile; -1 for an empty slot
+    short* lh = nullptr;    // [nslot][nh1] of H1 entry e (index into h1) in the tile that lists it
+    long long unresolved = 0;  // neighbours not found among the staged cells (0 unless the lists are wrong)
 };
+
+// The lists of one tile as the kernels see them, and the local index of cell j
+// among the tile's staged cells (T, then H1, then S2), or -1.
+struct TileLists {
+    int tb, tc, nh1, ngt, ns2, Nint;
+    const int* sH1;
+    const int* sS2;
+};
+
+__device__ __forceinline__ int lds_lower_bound(const int* a, int n, int x) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] < x) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ int local_of(const TileLists& L, int j) {
+    const unsigned d = (unsigned)(j - L.tb);
+    if (d < (unsigned)L.tc) return (int)d;
+    if (j < L.Nint) {
+        const int p = lds_lower_bound(L.sH1, L.nh1, j);
+        if (p < L.nh1 && L.sH1[p] == j) return L.tc + p;
+    } else {
+        const int p = lds_lower_bound(L.sS2, L.ngt, j);
+        if (p < L.ngt && L.sS2[p] == j) return L.tc + L.nh1 + p;
+    }
+    const int nr = L.ns2 - L.ngt;
+    const int p = lds_lower_bound(L.sS2 + L.ngt, nr, j);
+    if (p < nr && L.sS2[L.ngt + p] == j) return L.tc + L.nh1 + L.ngt + p;
+    return -1;
+}
 
 // Builds the tables from the neighbour planes cnb ([nslot][NcPad], neighbour |
 // twin << 29 or SLOT_EMPTY) of the Nint interior cells.  Tiles start as runs of
@@ -42,13 +82,13 @@ void build_halo_tables(int Nint, int NcPad, int nslot, const int* cnb, int TC, b
 void free_halo_tables(HaloTables& t);
 
 // Bytes of dynamic LDS the single-kernel step needs (block of 256 lanes):
-// staged states and face states (FP64: 32 B each, FP32-mixed: 16 B), the two
-// id lists and the tile-minimum slots (24 B, 8-aligned).
+// staged states and face states (FP64: 32 B each, FP32-mixed: 16 B), the
+// tile's CFL slots (24 B, in 32) and, in FP64, the H1 and S2 id lists it
+// searches for neighbours' local indices (FP32 reads them from lt / lh).
 inline size_t fused_lds_bytes(bool mixed, int maxdeg, int smax, int fsmax, int h1max, int s2max) {
     const size_t per = mixed ? 16 : 32;
-    size_t b = per * (size_t)smax + per * (size_t)maxdeg * fsmax;
-    b += 4 * (size_t)(((h1max + 3) & ~3) + ((s2max + 3) & ~3));
-    b += 32;
+    size_t b = per * (size_t)smax + per * (size_t)maxdeg * fsmax + 32;
+    if (!mixed) b += 4 * (size_t)(((h1max + 3) & ~3) + ((s2max + 3) & ~3));
     return b;
 }
 

@@ -43,6 +43,9 @@ struct PassArgs {
     const int* s_off = nullptr;
     const int* s_ngt = nullptr;
     const int* s2 = nullptr;
+    const short* lt = nullptr;          // [nslot][NcPad] local index of each neighbour in the cell's tile
+    const short* lh = nullptr;          // [nslot][nh1tot] the same for each H1 entry
+    long long nh1tot = 1;
     int smax = 0, fsmax = 0, h1max = 0, s2max = 0;
     double* tacc = nullptr;             // [ninst] simulated time
     double* lastdt = nullptr;           // [ninst] dt of the latest step

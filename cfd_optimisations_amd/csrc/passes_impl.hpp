@@ -136,7 +136,12 @@ __device__ __forceinline__ long long fs_index(long long P, int k, int i, int v) 
 // Geometry plane p of slot k of cell i (MAXDEG slot planes per quantity).
 template <int MAXDEG, typename T>
 __device__ __forceinline__ T geo(const T* __restrict__ g, long long P, int p, int k, int i) {
-    return g[(long long)(p * MAXDEG + k) * P + i];
+    return plane_at(g + (long long)(p * MAXDEG + k) * P, i);
+}
+
+// Slot k of cell i in the neighbour table.
+__device__ __forceinline__ int cnb_at(const PassArgs& a, int k, int i) {
+    return plane_at(a.cnb + (long long)k * a.NcPad, i);
 }
 
 // ---------------------------------------------------------------------------
@@ -149,13 +154,13 @@ template <>
 struct PassATypes<double> {
     __device__ static const double* geo(const PassArgs& a) { return a.cgeo; }
     __device__ static double* fs(const PassArgs& a) { return a.FS; }
-    __device__ static double iv(const PassArgs& a, int i) { return a.iv[i]; }
+    __device__ static double iv(const PassArgs& a, int i) { return plane_at(a.iv, i); }
 };
 template <>
 struct PassATypes<float> {
     __device__ static const float* geo(const PassArgs& a) { return a.cgeo32; }
     __device__ static float* fs(const PassArgs& a) { return a.FS32; }
-    __device__ static float iv(const PassArgs& a, int i) { return a.iv32[i]; }
+    __device__ static float iv(const PassArgs& a, int i) { return plane_at(a.iv32, i); }
 };
 
 // Tile staging: the block's cells [tb, tb + tc) have their state in LDS
@@ -184,7 +189,7 @@ __device__ __forceinline__ double pass_a_cell(const PassArgs& a, int i, const do
 #pragma unroll
     for (int v = 0; v < NV; v++) u[v] = (T)ud[v];
     const T iv = PT::iv(a, i);
-    const int side = a.cside[i];
+    const int side = plane_at(a.cside, i);
     T g[NV][2], umin[NV], umax[NV];
 #pragma unroll
     for (int v = 0; v < NV; v++) {
@@ -210,7 +215,7 @@ __device__ __forceinline__ double pass_a_cell(const PassArgs& a, int i, const do
     for (int k = 0; k < MAXDEG; k++) {
         r0[k] = T(0);
         r1[k] = T(0);
-        const int nbx = a.cnb[k * P + i];
+        const int nbx = cnb_at(a, k, i);
         if (nbx == SLOT_EMPTY) continue;
         deg = k + 1;
         const int nb = nbx & SLOT_NB_MASK;
@@ -271,7 +276,7 @@ __device__ __forceinline__ double pass_a_cell(const PassArgs& a, int i, const do
             store_fs(FS, P, k, i, uf);
             if ((ghosts >> k) & 1u) {
                 // the ghost side (R) of a boundary face: U_g + phi_L ((-G_L) . rR), rR = -rL
-                const int gc = a.cnb[k * P + i] & SLOT_NB_MASK;
+                const int gc = cnb_at(a, k, i) & SLOT_NB_MASK;
                 double ugd[NV];
                 tile_or_global_cell<W>(a, sU, tb, tc, gc, ugd);
                 T ufg[NV];
@@ -297,7 +302,7 @@ __device__ __forceinline__ double pass_a_cell_quad(const PassArgs& a, int i, int
     const double udv = a.U[cidx<W, NV>(i, v)];
     const T u = (T)udv;
     const T iv = PT::iv(a, i);
-    const int side = a.cside[i];
+    const int side = plane_at(a.cside, i);
     T g0 = T(0), g1 = T(0), umin = u, umax = u;
     double ux = 0, uy = 0, c = 0, lam = 0.0;
     if constexpr (CFL) {
@@ -318,7 +323,7 @@ __device__ __forceinline__ double pass_a_cell_quad(const PassArgs& a, int i, int
     for (int k = 0; k < MAXDEG; k++) {
         r0[k] = T(0);
         r1[k] = T(0);
-        const int nbx = a.cnb[k * P + i];
+        const int nbx = cnb_at(a, k, i);
         if (nbx == SLOT_EMPTY) continue;
         deg = k + 1;
         const int nb = nbx & SLOT_NB_MASK;
@@ -354,7 +359,7 @@ __device__ __forceinline__ double pass_a_cell_quad(const PassArgs& a, int i, int
             const T d = dot_r(g0, g1, r0[k], r1[k]);
             FS[fs_index<T>(P, k, i, v)] = madd(phi, d, u);
             if ((ghosts >> k) & 1u) {
-                const int gc = a.cnb[k * P + i] & SLOT_NB_MASK;
+                const int gc = cnb_at(a, k, i) & SLOT_NB_MASK;
                 const T ug = (T)a.U[cidx<W, NV>(gc, v)];
                 FS[fs_index<T>(P, 0, gc, v)] = madd(phi, d, ug);
             }
@@ -490,7 +495,7 @@ __device__ __forceinline__ void pass_b_cell_fp64(const PassArgs& a, int i, doubl
 #pragma unroll
     for (int k = 0; k < MAXDEG; k++) {
         gg[k] = -1;
-        const int nbx = a.cnb[k * P + i];
+        const int nbx = cnb_at(a, k, i);
         if (nbx == SLOT_EMPTY) continue;
         const int nb = nbx & SLOT_NB_MASK;
         const int tw = (int)((unsigned)nbx >> SLOT_TWIN_SHIFT);
@@ -556,7 +561,7 @@ __device__ __forceinline__ void pass_b_cell_mixed(const PassArgs& a, int i, doub
 #pragma unroll
     for (int k = 0; k < MAXDEG; k++) {
         gg[k] = -1;
-        const int nbx = a.cnb[k * P + i];
+        const int nbx = cnb_at(a, k, i);
         if (nbx == SLOT_EMPTY) continue;
         const int nb = nbx & SLOT_NB_MASK;
         const int tw = (int)((unsigned)nbx >> SLOT_TWIN_SHIFT);
