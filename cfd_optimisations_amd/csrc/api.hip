@@ -334,14 +334,12 @@ hcfd_ctx* create_ctx(int n, const hcfd_mesh_view* views, int precision, int flag
         build_halo_tables(c->Nint, c->NcPad, c->deg_bucket, c->m.cnb, 256, precision == HCFD_MIXED, budget, c->halo,
                           c->st);
     }
-    c->fused_lds = fused_lds_bytes(precision == HCFD_MIXED, c->deg_bucket, c->halo.smax, c->halo.fsmax,
-                                   c->halo.h1max, c->halo.s2max);
+    c->fused_lds = fused_lds_bytes(precision == HCFD_MIXED, c->deg_bucket, c->halo.smax, c->halo.fsmax);
     {
         int lim = 0;
         HIPCHK(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
         // runnable when the largest tile's halo fits in LDS with two blocks per CU
-        c->fused_ok = c->fused_lds <= (size_t)std::min(lim, 80 * 1024) &&
-                      (precision != HCFD_MIXED || (c->halo.lt && c->halo.unresolved == 0));
+        c->fused_ok = c->fused_lds <= (size_t)std::min(lim, 80 * 1024) && c->halo.lt && c->halo.unresolved == 0;
         // default only for compact tiles (at most 5% of the tiles had to be split):
         // strip-shaped tiles, as a reverse Cuthill-McKee order makes, recompute more
         // halo than they update; Hilbert tiles keep the halo near a quarter

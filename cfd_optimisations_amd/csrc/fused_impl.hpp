@@ -117,11 +117,7 @@ __device__ __forceinline__ void fused_cell_a(const PassArgs& a, const TileLists&
         if (nbx == SLOT_EMPTY) continue;
         deg = k + 1;
         const int nb = nbx & SLOT_NB_MASK;
-        // FP64: a search of the tile's lists in LDS (its registers are spoken
-        // for); FP32: the index read with the cell's inputs
-        int q;
-        if constexpr (sizeof(T) == 8) q = local_of(L, nb);
-        else q = c.lq[k];
+        int q = c.lq[k];  // the neighbour's place among the staged cells
         if (q < 0) {
             bad = true;
             q = l;
@@ -289,11 +285,6 @@ k_fused_step(
     V* sU = reinterpret_cast<V*>(fused_lds);
     V* sFS = sU + (size_t)FT::PER * su_pitch;
     TileMin* tmin = reinterpret_cast<TileMin*>(sFS + (size_t)FT::PER * MAXDEG * fs_pitch);
-    // FP64: the tile's H1 and S2 id lists, searched for neighbours' local
-    // indices, after the 32-byte CFL slots
-    constexpr bool search = sizeof(T) == 8;
-    int* sH1 = reinterpret_cast<int*>(reinterpret_cast<char*>(tmin) + 32);
-    int* sS2 = sH1 + ((a.h1max + 3) & ~3);
     const int tid = threadIdx.x;
     if (tid == 0) {
         tmin->first = 0ull;
@@ -304,7 +295,7 @@ k_fused_step(
     advance_clocks(a);  // dt of this step: fixed, or reduced from the CFL bound of U^n
     int tfirst, step, tend;
     tile_walk(a, tfirst, step, tend);
-    for (int tile = tfirst; tile < tend; tile += step) {
+    auto run_tile = [&](int tile) {
         TileLists L;
         L.tb = a.tile_start[tile];
         L.tc = a.tile_start[tile + 1] - L.tb;
@@ -313,8 +304,8 @@ k_fused_step(
         L.ns2 = a.s_off[tile + 1] - s0;
         L.ngt = a.s_ngt[tile];
         L.Nint = a.Nint;
-        L.sH1 = sH1;
-        L.sS2 = sS2;
+        L.sH1 = nullptr;  // neighbours' local indices come from lt / lh
+        L.sS2 = nullptr;
         const int i = L.tb + tid;
         const bool hasT = tid < L.tc;
         // instances of the tile's first and last cell (batches)
@@ -326,10 +317,9 @@ k_fused_step(
         // this lane's tile cell: its inputs are in flight while the tile is staged
         CellIn<T, MAXDEG> ci;
         double uown[NV] = {0, 0, 0, 0};  // U^n of the lane's tile cell (FP64: read back from LDS in pass B)
-        // (FP32: with its neighbours' local indices)
         if (hasT) {
             load_cell_in<T, MAXDEG>(a, i, ci);
-            if constexpr (!search) load_local(a.lt, a.NcPad, i, ci);
+            load_local(a.lt, a.NcPad, i, ci);
             load_cell<W, NV>(a.U, i, uown);
         }
         // the halo ring goes to one wavefront, rotated from tile to tile so the
@@ -342,7 +332,7 @@ k_fused_step(
         CellIn<T, MAXDEG> ch;
         if (early_ring && qh < L.nh1) {
             load_cell_in<T, MAXDEG>(a, a.h1[h0 + qh], ch);
-            if constexpr (!search) load_local(a.lh, a.nh1tot, h0 + qh, ch);
+            load_local(a.lh, a.nh1tot, h0 + qh, ch);
         }
         // staged halo cell q = tid (ring H1, then S2): its id and state are in
         // flight before the barrier too; further rounds (more than 256 staged
@@ -371,10 +361,6 @@ k_fused_step(
                 g = q < L.nh1 ? a.h1[h0 + q] : a.s2[s0 + q - L.nh1];
                 load_cell<W, NV>(a.U, g, ud);
             }
-            if constexpr (search) {
-                if (q < L.nh1) sH1[q] = g;
-                else sS2[q - L.nh1] = g;
-            }
             T ut[NV];
 #pragma unroll
             for (int v = 0; v < NV; v++) ut[v] = (T)ud[v];
@@ -390,7 +376,7 @@ k_fused_step(
         for (int q = qh; q < L.nh1; q += FUSED_TILE) {
             if (!early_ring || q != qh) {
                 load_cell_in<T, MAXDEG>(a, a.h1[h0 + q], ch);
-                if constexpr (!search) load_local(a.lh, a.nh1tot, h0 + q, ch);
+                load_local(a.lh, a.nh1tot, h0 + q, ch);
             }
             int ln[MAXDEG];
             fused_cell_a<T, MAXDEG>(a, L, ch, L.tc + q, false, sU, su_pitch, sFS, fs_pitch, ln, bad);
@@ -463,6 +449,14 @@ k_fused_step(
             a.fault[0] = 1;
         }
         if constexpr (CFL) cfl_tile_lds<MULTI>(a, tile, lane_inst, bf, bl, rate, tmin);
+    };
+    // One tile per block (the default grid) runs outside any loop: inside one,
+    // the compiler hoists the tile cell's LDS addresses out of it and, in FP64,
+    // spills them (measured 36-76 B per lane and 20% slower).
+    if (tfirst + step >= tend) {
+        if (tfirst < tend) run_tile(tfirst);
+    } else {
+        for (int tile = tfirst; tile < tend; tile += step) run_tile(tile);
     }
 }
 

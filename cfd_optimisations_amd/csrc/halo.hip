@@ -298,9 +298,9 @@ void build_halo_tables(int Nint, int NcPad, int nslot, const int* cnb, int TC, b
             out.fsmax = std::max(out.fsmax, tc + nh[t] + ng[t]);
             out.h1max = std::max(out.h1max, nh[t]);
             out.s2max = std::max(out.s2max, ns[t]);
-            bytes[t] = fused_lds_bytes(mixed, nslot, tc + nh[t] + ns[t], tc + nh[t] + ng[t], nh[t], ns[t]);
+            bytes[t] = fused_lds_bytes(mixed, nslot, tc + nh[t] + ns[t], tc + nh[t] + ng[t]);
         }
-        const size_t lds = fused_lds_bytes(mixed, nslot, out.smax, out.fsmax, out.h1max, out.s2max);
+        const size_t lds = fused_lds_bytes(mixed, nslot, out.smax, out.fsmax);
         if (lds <= lds_budget || round == 5) break;
         // halve the tiles whose own footprint is over budget (tightened a little
         // each round, since the block's LDS takes the maximum of each list)
@@ -323,7 +323,7 @@ void build_halo_tables(int Nint, int NcPad, int nslot, const int* cnb, int TC, b
         out.TC = TC;
     }
     out.split_tiles = out.ntiles - (Nint + TC - 1) / TC;
-    if (mixed) build_local_tables(Nint, NcPad, nslot, cnb, out, st);
+    build_local_tables(Nint, NcPad, nslot, cnb, out, st);
 }
 
 void free_halo_tables(HaloTables& t) {

@@ -9,10 +9,9 @@
 //          state), then the remaining neighbours of H1 cells (interior or ghost).
 // Each list is sorted by cell id.  A block stages the states of T, H1, S2 (in
 // that order: local index = position) and the face states of T, H1 and the
-// ghosts of T in LDS.  For FP32-mixed the local index of every neighbour of a
-// T or H1 cell is resolved once, at build time, into 16-bit tables (lt, lh),
-// so the step kernel reads it instead of searching the lists (in FP64 the
-// registers the table needs are not free, and the kernel searches).
+// ghosts of T in LDS.  The local index of every neighbour of a T or H1 cell is
+// resolved once, at build time, into 16-bit tables (lt, lh), so the step
+// kernel reads it with the cell's inputs instead of searching the lists.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -82,14 +81,11 @@ void build_halo_tables(int Nint, int NcPad, int nslot, const int* cnb, int TC, b
 void free_halo_tables(HaloTables& t);
 
 // Bytes of dynamic LDS the single-kernel step needs (block of 256 lanes):
-// staged states and face states (FP64: 32 B each, FP32-mixed: 16 B), the
-// tile's CFL slots (24 B, in 32) and, in FP64, the H1 and S2 id lists it
-// searches for neighbours' local indices (FP32 reads them from lt / lh).
-inline size_t fused_lds_bytes(bool mixed, int maxdeg, int smax, int fsmax, int h1max, int s2max) {
+// staged states and face states (FP64: 32 B each, FP32-mixed: 16 B) and the
+// tile's CFL slots (24 B, in 32).
+inline size_t fused_lds_bytes(bool mixed, int maxdeg, int smax, int fsmax) {
     const size_t per = mixed ? 16 : 32;
-    size_t b = per * (size_t)smax + per * (size_t)maxdeg * fsmax + 32;
-    if (!mixed) b += 4 * (size_t)(((h1max + 3) & ~3) + ((s2max + 3) & ~3));
-    return b;
+    return per * (size_t)smax + per * (size_t)maxdeg * fsmax + 32;
 }
 
 }  // namespace hcfd
