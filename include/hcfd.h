@@ -110,7 +110,8 @@ typedef struct hcfd_info {
     int ncolours;        /* face colours of the conflict-free colouring */
     long long device_bytes;
     hcfd_schedule sched;
-    int fused_available;  /* the halo tiles of this mesh fit in LDS (see hcfd_schedule.fused); the
+    int fused_available;  /* the halo tiles of this mesh fit in LDS and their local-index tables
+                             were built (see hcfd_schedule.fused); the
                              single-kernel step is the default schedule when, in addition,
                              at most 5% of the 256-cell tiles had to be split to fit */
     int fused_lds_bytes;  /* LDS per block of the single-kernel step */
