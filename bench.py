@@ -151,7 +151,7 @@ def build_case(name: str):
 
 
 def run_single(name: str, steps: int, warmup: int, cpu_s: float, graph: bool, secondary: bool, tune: bool,
-               order: str = "sfc"):
+               order: str = "sfc", step_path: str = "default"):
     import numpy as np
     from cfd_optimisations_amd import cases as C
     from cfd_optimisations_amd import lib as L
@@ -165,6 +165,8 @@ def run_single(name: str, steps: int, warmup: int, cpu_s: float, graph: bool, se
     inf = ctx.info()
     if tune:
         ctx.autotune(steps=10, cfl=cfl, cache=os.path.join(tempfile.gettempdir(), "hcfd_tune.txt"))
+    if step_path != "default":
+        ctx.set_schedule(fused=1 if step_path == "single" else 0)
     # warm-up steps, then the state is put back: the timed steps are steps
     # 1..steps of the workload's run from its initial state (the Riemann problem
     # is quoted on 500 steps and leaves the physical range some steps later)
@@ -521,6 +523,8 @@ def main(argv=None) -> int:
     ap.add_argument("--tune", action="store_true", help="run the autotuner before timing")
     ap.add_argument("--no-tune", action="store_true", help="batches: keep the default schedule")
     ap.add_argument("--no-secondary", action="store_true")
+    ap.add_argument("--step-path", default="default", choices=["default", "single", "two-pass"],
+                    help="single meshes: force the single-kernel step or the two passes")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {a.gpus}).",
@@ -534,7 +538,7 @@ def main(argv=None) -> int:
         line = run_batch(a.workload, a.steps, a.warmup, not a.no_tune, a.check, not a.no_compare)
     else:
         line = run_single(a.workload, a.steps, a.warmup, a.cpu_seconds, a.graph, not a.no_secondary, a.tune,
-                          a.order)
+                          a.order, a.step_path)
     print(json.dumps(line))
     return 0
 

@@ -132,6 +132,29 @@ def test_wedge_reference_case_schedules(wedge_mesh, sched):
         assert np.array_equal(ctx.get_state(), Uo)
 
 
+@pytest.mark.parametrize("order", [0, L.FLAG_ORDER_SFC, L.FLAG_NO_RENUMBER], ids=["rcm", "hilbert", "caller"])
+def test_single_kernel_every_cell_order(order):
+    """The single kernel's tiles (and the local-index tables resolved for them)
+    in each cell order: reverse Cuthill-McKee strips, Hilbert blocks and the
+    generator's own row order.  FP64 equals the oracle bit for bit; FP32-mixed
+    equals its two-pass schedule bit for bit."""
+    case = C.riemann_case(48, 36, seed=3, shuffle=False)
+    od, U0 = oracle_for(case)
+    Uo, to = od.run(U0, 40, cfl=0.45)
+    with C.make_context(case, L.FP64, order) as ctx:
+        assert ctx.info().fused_available
+        ctx.set_schedule(**FUSED)
+        assert ctx.step(40, cfl=0.45) == to
+        assert np.array_equal(ctx.get_state(), Uo)
+        assert ctx.fault() == 0
+    with C.make_context(case, L.MIXED, order) as a, C.make_context(case, L.MIXED, order) as b:
+        a.set_schedule(**TWO_PASS)
+        b.set_schedule(**FUSED)
+        assert a.step(40, cfl=0.45) == b.step(40, cfl=0.45)
+        assert np.array_equal(a.get_state(), b.get_state())
+        assert relerr(b.get_state(), Uo)[0] < MIXED_RHO_TOL
+
+
 def test_fixed_dt_after_cfl_steps():
     """Fixed dt, then CFL steps, then the same fixed dt again: the CFL reduction
     overwrites the device dt, which must be reloaded; the clock runs on through
