@@ -292,3 +292,14 @@ def test_vtu_export_matches_mesh_and_solution(tmp_path, wedge_mesh):
     _, _, b = read_vtu(str(tmp_path / "r.vtu"))
     assert (b["types"] == 5).sum() == 30 and (b["types"] == 3).sum() == hm.Nc - 30
     assert np.array_equal(b["Density"], Q2[0])
+
+
+def test_bench_refuses_several_gpus():
+    """bench.py measures one GPU: asked for more it says so and exits with status 2,
+    before touching a device."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout == ""
