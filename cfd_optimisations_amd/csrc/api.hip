@@ -465,8 +465,6 @@ Launch make_launch(hcfd_ctx* c, bool cfl, int cur, int pass) {
     a.nh1tot = std::max(c->halo.nh1, 1LL);
     a.smax = c->halo.smax;
     a.fsmax = c->halo.fsmax;
-    a.h1max = c->halo.h1max;
-    a.s2max = c->halo.s2max;
     a.tacc = c->tdev;
     a.lastdt = c->lastdt;
     L.block = fused ? 256 : s.block;

@@ -46,7 +46,7 @@ struct PassArgs {
     const short* lt = nullptr;          // [nslot][NcPad] local index of each neighbour in the cell's tile
     const short* lh = nullptr;          // [nslot][nh1tot] the same for each H1 entry
     long long nh1tot = 1;
-    int smax = 0, fsmax = 0, h1max = 0, s2max = 0;
+    int smax = 0, fsmax = 0;
     double* tacc = nullptr;             // [ninst] simulated time
     double* lastdt = nullptr;           // [ninst] dt of the latest step
     int advance = 0;                    // first stage of a step: block 0 of pass B (or of the single
