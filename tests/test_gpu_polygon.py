@@ -141,3 +141,19 @@ def test_polygon_cells_mixed():
             b.set_schedule(**FUSED)
             b.step(40, cfl=0.5)
             assert np.array_equal(b.get_state(), Ua)
+
+
+@pytest.mark.parametrize("sched", [TWO_PASS, FUSED], ids=["two_pass", "fused"])
+def test_polygon_cells_ssprk3_exact(sched):
+    """SSP-RK3 on the same mesh (the stage combination in the eight-slot kernels):
+    equal to the oracle's Shu-Osher steps bit for bit."""
+    case = polygon_case()
+    od, U0 = oracle_for(case)
+    Uo, to = od.run_ssprk(U0, 15, 3, cfl=0.5)
+    with C.make_context(case, L.FP64, L.FLAG_ORDER_SFC) as ctx:
+        if sched["fused"] and not ctx.info().fused_available:
+            pytest.skip("the single-kernel tiles of this mesh do not fit in LDS")
+        ctx.set_schedule(**sched)
+        ctx.set_integrator(3)
+        assert ctx.step(15, cfl=0.5) == to
+        assert np.array_equal(ctx.get_state(), Uo)
