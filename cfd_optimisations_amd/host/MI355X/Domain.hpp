@@ -28,10 +28,11 @@ class Domain {
     double* iv;
 
     // precision: HCFD_FP64 or HCFD_MIXED; flags: hcfd_flags.  The FP64 step is
-    // bit-identical to Base in every cell order; the default Hilbert order gives
-    // the compact tiles the single-kernel step needs (HCFD_FLAG_EXACT alone keeps
-    // reverse Cuthill-McKee).
-    explicit Domain(Mesh& mesh, int precision = HCFD_FP64, int flags = HCFD_FLAG_EXACT | HCFD_FLAG_ORDER_SFC);
+    // bit-identical to Base in every cell order (HCFD_FLAG_EXACT only restates
+    // that, and is refused with HCFD_MIXED); the default Hilbert order gives the
+    // compact tiles the single-kernel step needs (flags 0 keep reverse
+    // Cuthill-McKee).
+    explicit Domain(Mesh& mesh, int precision = HCFD_FP64, int flags = HCFD_FLAG_ORDER_SFC);
     ~Domain();
     Domain(const Domain&) = delete;
     Domain& operator=(const Domain&) = delete;

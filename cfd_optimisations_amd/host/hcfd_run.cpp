@@ -2,7 +2,7 @@
 // through the MI355X drop-in classes:
 //
 //   hcfd_run [mesh.hmsh] [output stem] [steps] [--stages] [--outputs K] [--rk S]
-//            [--restart file.hsln]
+//            [--restart file.hsln] [--mixed]
 //
 // Default: ../wedge.hmsh, ../sols/MI355X, 1000 steps of dt = 1e-3, writing
 // <stem>_0.hsln and <stem>_1.hsln as the reference writes sols/Base_{0,1}.hsln.
@@ -10,7 +10,9 @@
 // (GaussGreen, Limiter, BC copies, Flux, update, BC cells) instead of the
 // fused device step.  --outputs K writes K files after the initial one
 // (dt_out = T / K); --rk S steps with S-stage SSP-RK; --restart starts from a
-// .hsln file instead of the uniform initial state.
+// .hsln file instead of the uniform initial state; --mixed builds the Domain
+// for the FP32-mixed path (MI355X::Domain(mesh, HCFD_MIXED); the per-stage
+// classes of --stages are FP64 only).
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -33,7 +35,7 @@ int main(int argc, char** argv) {
         const char* mesh_path = argc > 1 ? argv[1] : "../wedge.hmsh";
         const std::string out = argc > 2 ? argv[2] : "../sols/MI355X";
         const int niter = argc > 3 ? std::atoi(argv[3]) : 1000;
-        bool stages = false;
+        bool stages = false, mixed = false;
         int nout = 1, rk = 1;
         std::string restart;
         for (int k = 4; k < argc; k++) {
@@ -41,9 +43,11 @@ int main(int argc, char** argv) {
             else if (std::strcmp(argv[k], "--outputs") == 0 && k + 1 < argc) nout = std::atoi(argv[++k]);
             else if (std::strcmp(argv[k], "--rk") == 0 && k + 1 < argc) rk = std::atoi(argv[++k]);
             else if (std::strcmp(argv[k], "--restart") == 0 && k + 1 < argc) restart = argv[++k];
+            else if (std::strcmp(argv[k], "--mixed") == 0) mixed = true;
             else throw std::runtime_error(std::string("unknown option ") + argv[k]);
         }
         if (nout < 1) throw std::runtime_error("--outputs must be at least 1");
+        if (mixed && stages) throw std::runtime_error("--stages runs the FP64 per-stage classes; drop --mixed");
 
         // the case of src/Test.cpp:51-64
         const int nvars = 4;
@@ -57,7 +61,7 @@ int main(int argc, char** argv) {
                                         MI355X::SUPOUTLET};
 
         Mesh mesh(mesh_path);
-        MI355X::Domain domain(mesh);
+        MI355X::Domain domain(mesh, mixed ? HCFD_MIXED : HCFD_FP64);
         MI355X::GaussGreen grads(domain);
         MI355X::Limiter limiter(domain);
         MI355X::Flux flux(nvars, gma, domain);

@@ -115,3 +115,19 @@ def test_dropin_ssprk_and_restart(tmp_path, wedge_mesh):
     r = subprocess.run([RUN, os.path.join(GOLDEN, "wedge.hmsh"), str(tmp_path / "x"), "1", "--restart",
                         str(tmp_path / "missing.hsln")], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "cannot open" in r.stderr
+
+
+def test_dropin_mixed_domain_within_bound(tmp_path, wedge_mesh):
+    """MI355X::Domain(mesh, HCFD_MIXED) through the drop-in Euler: src/Test.cpp's
+    1000 steps stay within the mixed path's stated bound of the oracle (relative
+    L2 of the density 1e-5), and the per-stage classes refuse the mixed Domain."""
+    stem = str(tmp_path / "mixed")
+    _run([os.path.join(GOLDEN, "wedge.hmsh"), stem, 1000, "--mixed"])
+    _, Q1 = oracle_prims(wedge_mesh, 1000)
+    _, S1 = read_hsln(stem + "_1.hsln")
+    err = [np.linalg.norm(S1[v] - Q1[v]) / np.linalg.norm(Q1[v]) for v in range(4)]
+    assert err[0] < 1e-5 and max(err) < 1e-4, err
+    assert not np.array_equal(S1, Q1)  # the mixed arithmetic did run
+    r = subprocess.run([RUN, os.path.join(GOLDEN, "wedge.hmsh"), str(tmp_path / "x"), "1", "--stages", "--mixed"],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "FP64" in r.stderr
