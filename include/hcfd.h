@@ -166,7 +166,8 @@ int hcfd_get_prims(hcfd_ctx* ctx, double* const Q[4]);
 
 /* ---- per-stage entry points (drop-in parity with the Base classes) -------- */
 /* These act on device-resident stage arrays; the hcfd_stage_get/set calls
- * move them.  which: 0 = U, 1 = G ([4][2*Nc]), 2 = phi, 3 = R. */
+ * move them.  which: 0 = U, 1 = G ([4][2*Nc]), 2 = phi, 3 = R.  They are the
+ * FP64 Base methods: an HCFD_MIXED context returns HCFD_ERR_STATE. */
 enum hcfd_array { HCFD_ARR_U = 0, HCFD_ARR_G = 1, HCFD_ARR_PHI = 2, HCFD_ARR_R = 3 };
 int hcfd_stage_set(hcfd_ctx* ctx, int which, const double* const X[4]);
 int hcfd_stage_get(hcfd_ctx* ctx, int which, double* const X[4]);
