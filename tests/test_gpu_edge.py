@@ -58,3 +58,23 @@ def test_zero_steps_change_nothing():
         assert np.array_equal(ctx.get_state(), U3)
         with pytest.raises(L.HcfdError):
             ctx.step(-1, cfl=0.5)
+
+
+def test_mixed_batch_with_tiny_instances():
+    """FP32-mixed on the same ragged batch: the single kernel equals the two passes
+    bit for bit per instance, and each instance stays within the mixed bound of its
+    FP64 oracle run."""
+    from test_gpu_parity import MIXED_RHO_TOL, relerr
+    cs = [C.ramp_case(1, 1, 2.0, 10.0, seed=6), C.ramp_case(12, 6, 2.5, 12.0, seed=7),
+          C.ramp_case(2, 1, 1.8, 5.0, seed=8)]
+    with C.make_batch(cs, L.MIXED, L.FLAG_ORDER_SFC) as a, C.make_batch(cs, L.MIXED, L.FLAG_ORDER_SFC) as b:
+        a.set_schedule(**TWO_PASS)
+        b.set_schedule(**FUSED)
+        a.step(12, cfl=0.3)
+        b.step(12, cfl=0.3)
+        for k, c in enumerate(cs):
+            Ua = a.inst_state(k)
+            assert np.array_equal(b.inst_state(k), Ua), k
+            od, U0 = oracle_for(c)
+            Uo, _ = od.run(U0, 12, cfl=0.3)
+            assert relerr(Ua, Uo)[0] < MIXED_RHO_TOL, k
