@@ -436,3 +436,60 @@ def test_ssprk_temporal_order():
         assert max(rates) < stages + 1.2, (stages, rates)
     with pytest.raises(ValueError):
         od.run_ssprk(U0, 1, 4, dt=1e-3)
+
+
+# ---------------------------------------------------------------------------
+# orc_initialise_prims: per-cell primitive initial state (builds the Riemann,
+# restart and developed-flow cases), pinned to the golden-pinned uniform form
+# ---------------------------------------------------------------------------
+def _random_physical_prims(rng, n):
+    rho = 0.1 + 2.0 * rng.random(n)
+    u = rng.uniform(-3.0, 3.0, n)
+    v = rng.uniform(-3.0, 3.0, n)
+    p = 0.02 + 2.0 * rng.random(n)
+    return np.stack([rho, u, v, p])
+
+
+def test_initialise_prims_pinned(wedge):
+    """Euler::initialise (src/Base/Euler.cpp:62-77) applied cell by cell.
+
+    * A uniform Q gives exactly orc_initialise (which sols/Base_0.hsln pins),
+      also for states other than the golden one.
+    * A piecewise-constant Q of five states gives, on each interior cell, the
+      bits orc_initialise gives for that cell's state, and on the ghosts the
+      BC update (pinned against the reference's update_cells) of that interior.
+    * Permuting Q over the interior permutes the interior of U exactly.
+    * compute_prims of the result returns a random physical Q to a few ulps.
+    """
+    m, od, qic, qbc, bct, _ = wedge
+    ghost = m.ghost_mask()
+    inter = ~ghost
+    rng = np.random.default_rng(11)
+    for q in [qic, np.array([1.3, -0.4, 2.1, 0.7]), np.array([0.138, 1.206, 1.206, 0.029])]:
+        Q = np.tile(q[:, None], (1, m.Nc))
+        assert np.array_equal(od.initialise_prims(Q), od.initialise(q))
+    states = _random_physical_prims(rng, 5)
+    lab = rng.integers(0, 5, m.Nc)
+    Q = states[:, lab]
+    U = od.initialise_prims(Q)
+    Uexp = np.zeros_like(U)
+    for s in range(5):
+        Us = od.initialise(states[:, s])
+        sel = inter & (lab == s)
+        Uexp[:, sel] = Us[:, sel]
+    Uexp = od.bc_cells(Uexp)
+    assert np.array_equal(U, Uexp)
+    # the same cell-local arithmetic on a random per-cell Q, permuted over the interior
+    Q = _random_physical_prims(rng, m.Nc)
+    U = od.initialise_prims(Q)
+    idx = np.flatnonzero(inter)
+    perm = idx[rng.permutation(idx.size)]
+    Qp = Q.copy()
+    Qp[:, idx] = Q[:, perm]
+    Up = od.initialise_prims(Qp)
+    assert np.array_equal(Up[:, idx], U[:, perm])
+    # prim -> cons -> prim round trip on the interior (ghosts hold BC states)
+    Qb = od.prims(U)
+    rel = np.abs(Qb[:, inter] - Q[:, inter]) / np.maximum(np.abs(Q[:, inter]), 1e-300)
+    assert rel[0].max() == 0.0  # density is copied
+    assert rel[1:3].max() < 4e-16 * 4 and rel[3].max() < 1e-12
