@@ -1,17 +1,21 @@
 #!/usr/bin/env python3
 """Benchmark of the explicit unstructured finite-volume Euler residual on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c3|c3mixed|c4|c5|c2rk|c2out] [--check N]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c5|c4|c2|c3|c3mixed|c2rk|c2out] [--check N]
 
-c2 (default) is the 4M-triangle FP64 workload; c3/c3mixed the 16M-triangle ramp; c4/c5
-the batches; c2rk the SSP-RK integrators' cost; c2out the output path (files written
-while stepping).
+c5 (default): 1,024 wedge instances of uneven size, Mach number and angle in one
+batch, FP32-mixed, with the same batch in FP64 beside it: both throughputs, their
+ratio and the per-instance density error of mixed against FP64 after the quoted
+2,000 steps; the C2 (4M-triangle Riemann, FP64) and C3 (16M-triangle ramp, FP64 and
+mixed) single meshes as secondary lines.  c4 is the 256-wedge FP64 batch, c2/c3/c3mixed
+the single meshes alone, c2rk the SSP-RK integrators' cost, c2out the output path
+(files written while stepping).
 
-A step is one residual evaluation plus the forward-Euler update (src/Base/Euler.cpp:85-97):
-pass A (gradient + limiter + CFL bound), the CFL reduction, pass B (flux + update +
-ghost BCs).  The state is resident in HBM before the timed region starts.  Prints ONE
-JSON line with cell-updates/s (interior cells x steps / s), the HBM roofline of the
-dominant kernel and the reference's CPU solver timed on this host on a bounded sample.
+A step is one residual evaluation plus the forward-Euler update (src/Base/Euler.cpp:85-97),
+with the CFL time step.  The state is resident in HBM before the timed region starts;
+exactly --steps steps are timed after --warmup untimed ones.  Prints ONE JSON line with
+cell-updates/s (interior cells x steps / s), the HBM roofline of the dominant kernel
+and the reference's CPU solver timed on this host on a bounded sample.
 """
 from __future__ import annotations
 
@@ -49,7 +53,8 @@ def alg_bytes(Nc: int, Nf: int, mixed: bool):
     return a, b, a + b
 
 
-PMC_SUMMARY = {"c2": "pmc_riemann_4m_fp64.md", "c3": "pmc_ramp_16m_fp64.md", "c3mixed": "pmc_ramp_16m_mixed.md"}
+PMC_SUMMARY = {"c2": "pmc_riemann_4m_fp64.md", "c3": "pmc_ramp_16m_fp64.md", "c3mixed": "pmc_ramp_16m_mixed.md",
+               "c4": "pmc_wedges_256_fp64.md", "c5": "pmc_wedges_1024_mixed.md"}
 
 
 def read_traffic(workload: str):
@@ -225,41 +230,65 @@ def run_single(name: str, steps: int, warmup: int, cpu_s: float, graph: bool, se
         line["cpu_baseline"] = None
     ctx.close()
     if secondary and name == "c2":
-        line["secondary"] = secondary_c3(steps=None)
+        line["secondary"] = [r for r in secondary_lines() if not r["workload"].startswith("c2")]
     return line
 
 
-def secondary_c3(steps=None):
-    """FP64 and FP32-mixed on the 16M-triangle ramp: throughput and the mixed path's
-    error per conserved variable against the FP64 GPU result after the same steps."""
+def secondary_lines():
+    """The single-mesh workloads beside the batch: C2 (4M-triangle Riemann, FP64, its
+    quoted 500 steps from the initial state) and C3 (16M-triangle ramp, 1000 steps,
+    FP64 and FP32-mixed), each timed over its quoted steps with its own roofline
+    (event-timed dominant kernel, traffic from the committed rocprofv3 --pmc
+    summary), and the C3 mixed run's error per conserved variable against the C3
+    FP64 run after the same steps."""
     import numpy as np
     from cfd_optimisations_amd import cases as C
     from cfd_optimisations_amd import lib as L
-    w = C.WORKLOADS["c3_ramp_16m"]
-    steps = steps or w["steps"]
-    case = C.ramp_case(w["nx"], w["ny"], w["mach"], w["angle"], seed=0)
     out = []
     states = {}
-    for prec in (L.FP64, L.MIXED):
+    for name in ("c2", "c3", "c3mixed"):
+        case, w, prec, desc = build_case(name)
+        steps, cfl = w["steps"], w["cfl"]
         ctx = C.make_context(case, prec, L.FLAG_ORDER_SFC)
         inf = ctx.info()
-        tm = ctx.time_steps(steps, cfl=w["cfl"])
-        _, _, step_b = alg_bytes(inf.Nc, inf.Nf, prec == L.MIXED)
-        ms = tm["wall_ms"] / steps
-        rec = {"workload": f"c3 ramp {w['nx']}x{w['ny']} quads, Mach {w['mach']}, {w['angle']} deg, "
-                           f"{'FP32-mixed' if prec else 'FP64'}, {steps} steps CFL {w['cfl']}",
-               "value": inf.Nint * steps / (tm["wall_ms"] * 1e-3), "unit": "cell-updates/s", "ms_per_step": ms,
-               "schedule": ctx.schedule(), "kernel_ms": tm["pass_a_ms"] + tm["pass_b_ms"],
-               "hbm_pct_step": 100.0 * step_b / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), "fault": ctx.fault()}
-        states[prec] = ctx.get_state()
+        U_init = ctx.get_state()
+        ctx.step(3, cfl=cfl)            # warm-up, then the initial state again
+        ctx.set_state(U_init)
+        ctx.fault()
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.step(steps, cfl=cfl)
+        wall = time.perf_counter() - t0
+        fault = ctx.fault()
+        if name != "c2":
+            states[prec] = ctx.get_state()
+        ctx.set_state(U_init)
+        nev = min(steps, 100)
+        tm = ctx.time_steps(nev, cfl=cfl)
+        a_b, b_b, step_b = alg_bytes(inf.Nc, inf.Nf, prec == L.MIXED)
+        fused = bool(ctx.schedule()["fused"])
+        kern = {"fused_step": (tm["pass_a_ms"], step_b)} if fused else \
+            {"pass_a": (tm["pass_a_ms"], a_b), "pass_b": (tm["pass_b_ms"], b_b)}
+        dom = max(kern, key=lambda k: kern[k][0])
+        achieved = kern[dom][1] / (kern[dom][0] * 1e-3) / 1e9
+        ms = 1e3 * wall / steps
+        out.append({"workload": desc + f", {steps} steps from the initial state",
+                    "value": inf.Nint * steps / wall, "unit": "cell-updates/s", "ms_per_step": ms,
+                    "dtype": "f32+f64" if prec == L.MIXED else "f64", "schedule": ctx.schedule(),
+                    "kernel_ms": {dom: kern[dom][0], "dt_reduce": tm["dt_reduce_ms"]},
+                    "hbm_pct_step": 100.0 * step_b / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9),
+                    "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                                 "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "alg_bytes": kern[dom][1],
+                                 "traffic": (read_traffic(name) or {}).get(dom)},
+                    "fault": fault})
+        log(f"{name}: {ms:.4f} ms per step, {dom} {kern[dom][0]:.4f} ms ({achieved / HBM_PEAK_GBS:.3f} of peak)")
         ctx.close()
-        out.append(rec)
     Ua, Um = states[L.FP64], states[L.MIXED]
     names = ("rho", "rho_u", "rho_v", "rho_E")
-    out[1]["error_vs_fp64"] = {
+    out[2]["error_vs_fp64"] = {
         names[v]: {"max_abs": float(np.max(np.abs(Um[v] - Ua[v]))),
                    "rel_l2": float(np.linalg.norm(Um[v] - Ua[v]) / np.linalg.norm(Ua[v]))} for v in range(4)}
-    out[1]["speedup_vs_fp64"] = out[1]["value"] / out[0]["value"]
+    out[2]["speedup_vs_fp64"] = out[2]["value"] / out[1]["value"]
     return out
 
 
@@ -361,15 +390,92 @@ def log(msg: str) -> None:
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
-def run_batch(name: str, steps: int, warmup: int, tune: bool, check: int, compare_fp64: bool):
-    """Batched wedges (one launch per pass for all instances, hipGraph replay of the
-    steps, autotuned schedule).  value = interior cells of all instances x steps / wall."""
+def batch_cpu_baseline(cases, specs, order, dts, target_s: float, threads: int):
+    """The reference's own VEC layout (oracle/_ref, AVX2 build), one instance per
+    thread (the ctypes calls release the interpreter lock, so the threads run on
+    separate cores): the `threads` instances nearest the batch's median size,
+    each stepped with the fixed dt its GPU run ended with for about target_s
+    seconds after one warm-up step.  value = interior cells x steps summed over
+    the instances / wall time of the parallel section."""
+    import concurrent.futures as cf
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+    from cfd_optimisations_amd.formats import write_hmsh
+    if not ol.ref_available(ol.REF_VEC_SO):
+        return None
+    sizes = np.array([c.n_interior() for c in cases])
+    pick = [int(k) for k in np.argsort(np.abs(sizes - np.median(sizes)))[:threads]]
+    td = tempfile.mkdtemp()
+    jobs = []
+    for k in pick:
+        c = cases[k]
+        hm = c.hmesh()
+        path = os.path.join(td, f"inst{k}.hmsh")
+        write_hmsh(path, hm)
+        vc = ol.VecRefCase(path, c.gamma, [0] + [int(b) for b in c.bctypes], c.qbc)
+        q = c.qic
+        U0 = np.zeros((4, hm.Nc))
+        U0[0], U0[1], U0[2] = q[0], q[0] * q[1], q[0] * q[2]
+        U0[3] = q[3] / (c.gamma - 1) + 0.5 * q[0] * (q[1] ** 2 + q[2] ** 2)
+        jobs.append((k, vc, U0, float(dts[k])))
+
+    def probe(job):
+        k, vc, U0, dt = job
+        t0 = time.perf_counter()
+        U = vc.run(U0, 1, dt)
+        return k, U, time.perf_counter() - t0
+
+    with cf.ThreadPoolExecutor(len(jobs)) as ex:
+        first = {k: (U, sec) for k, U, sec in ex.map(probe, jobs)}
+        nsteps = {k: max(1, int(target_s / max(first[k][1], 1e-6))) for k, _, _, _ in jobs}
+
+        def timed(job):
+            k, vc, _, dt = job
+            vc.run(first[k][0], nsteps[k], dt)
+            return k
+
+        t0 = time.perf_counter()
+        list(ex.map(timed, jobs))
+        wall = time.perf_counter() - t0
+    for _, vc, _, _ in jobs:
+        vc.close()
+    for k in pick:
+        os.remove(os.path.join(td, f"inst{k}.hmsh"))
+    os.rmdir(td)
+    work = sum(int(sizes[k]) * nsteps[k] for k in pick)
+    cpu = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"value": work / wall, "unit": "cell-updates/s", "cores": len(jobs), "kind": "reference",
+            "sample": f"reference VEC layout (AVX2 build, oracle/_ref, FP64: the reference has no other precision), "
+                      f"one instance per thread on {len(jobs)} threads of {cpu}: the {len(jobs)} instances nearest "
+                      f"the batch's median size ({int(np.median(sizes)):,} interior cells), each stepped "
+                      f"{min(nsteps.values())}-{max(nsteps.values())} steps of its GPU run's last dt after one "
+                      f"warm-up step from its free stream",
+            "seconds": round(wall, 3), "instances": pick}
+
+
+def run_batch(name: str, steps: int, warmup: int, tune: bool, check: int, compare_fp64: bool, err_steps: int,
+              sensitivity: bool, cpu_s: float, secondary: bool, force_width: int = 0):
+    """Batched wedges (one launch per step for all instances, hipGraph replay,
+    autotuned schedule).  value = interior cells of all instances x steps / wall
+    over exactly `steps` steps after `warmup`.  Before the timed steps each batch
+    runs the workload's quoted step count (err_steps) from its initial state, so
+    the FP32-mixed batch can be compared with the FP64 batch after the same steps;
+    those runs are outside the timed region."""
     import numpy as np
     from cfd_optimisations_amd import cases as C
     from cfd_optimisations_amd import lib as L
     key = {"c4": "c4_batch_fp64", "c5": "c5_batch_mixed"}[name]
     w = C.WORKLOADS[key]
     steps = steps or w["steps"]
+    err_steps = w["steps"] if err_steps is None else err_steps
     cfl = w["cfl"]
     prec = w["precision"]
     t0 = time.perf_counter()
@@ -379,7 +485,11 @@ def run_batch(name: str, steps: int, warmup: int, tune: bool, check: int, compar
     log(f"{name}: generated {len(cases)} meshes in {gen_s:.1f} s")
     flags = L.FLAG_GRAPH | L.FLAG_ORDER_SFC
 
-    def one(precision, perturb=0.0, sched=None):
+    def one(precision, perturb=0.0, sched=None, timed=True, keep=None):
+        """Builds the batch, runs err_steps steps from the initial state (untimed),
+        then `warmup` + exactly `steps` timed steps; returns the context, its record,
+        the per-instance states after the err_steps steps (variables in `keep`)
+        and the initial states of the `check` smallest instances."""
         t1 = time.perf_counter()
         ctx = C.make_batch(cases, precision, flags)
         setup = time.perf_counter() - t1
@@ -394,33 +504,50 @@ def run_batch(name: str, steps: int, warmup: int, tune: bool, check: int, compar
             log(f"initial states perturbed by {perturb:g} (relative)")
         order = np.argsort(ctx.inst_sizes)
         U0 = {int(k): ctx.inst_state(int(k)) for k in order[:check]}
+        if sched is None and force_width:
+            sched = ctx.schedule() | {"aosoa_width": force_width}
         if sched is not None:
             ctx.set_schedule(**sched)
         else:
             sched = ctx.autotune(steps=4, cfl=cfl) if tune else ctx.schedule()
         log(f"schedule {sched}")
-        ctx.step(warmup, cfl=cfl)
-        t1 = time.perf_counter()
-        ctx.step(steps, cfl=cfl)          # graph replays; returns after the stream drained
-        wall = time.perf_counter() - t1
-        log(f"{steps} steps in {wall:.2f} s")
-        nev = min(steps, 20)
-        tm = ctx.time_steps(nev, cfl=cfl)   # per-kernel event timing, eager launches
-        tb, dtb, fb = ctx.status()
-        rec = {"value": inf.Nint * steps / wall, "ms_per_step": 1e3 * wall / steps, "Nc": inf.Nc,
-               "Nint": inf.Nint, "Nf": inf.Nf, "setup_s": round(setup, 2), "schedule": sched,
-               "kernel_ms": ({"fused_step": tm["pass_a_ms"]} if sched["fused"] else
-                             {"pass_a": tm["pass_a_ms"], "pass_b": tm["pass_b_ms"]}) |
-                            {"dt_reduce": tm["dt_reduce_ms"]},
-               "faulted_instances": int(fb.sum()),
-               "faulted": [{"instance": int(k), "mach": round(specs[k]["mach"], 3),
-                            "angle": round(specs[k]["angle"], 2), "triangles": 2 * specs[k]["nx"] * specs[k]["ny"]}
-                           for k in np.nonzero(fb)[0]],
-               "t_range": [float(tb[fb == 0].min()), float(tb[fb == 0].max())] if (fb == 0).any() else None,
-               "device_bytes": inf.device_bytes, "steps_total": warmup + steps + nev}
-        return ctx, rec, U0
+        rec = {"setup_s": round(setup, 2), "schedule": sched, "Nc": inf.Nc, "Nint": inf.Nint, "Nf": inf.Nf,
+               "device_bytes": inf.device_bytes}
+        states = None
+        if err_steps:
+            t1 = time.perf_counter()
+            ctx.step(err_steps, cfl=cfl)
+            log(f"{err_steps} steps from the initial state in {time.perf_counter() - t1:.1f} s (untimed)")
+            tb, dtb, fb = ctx.status()
+            rec["err_run"] = {"steps": err_steps, "faulted_instances": int(fb.sum()),
+                              "faulted": [{"instance": int(k), "mach": round(specs[k]["mach"], 3),
+                                           "angle": round(specs[k]["angle"], 2),
+                                           "triangles": 2 * specs[k]["nx"] * specs[k]["ny"]}
+                                          for k in np.nonzero(fb)[0]]}
+            rec["last_dt"] = dtb
+            if keep is not None:
+                states = [ctx.inst_state(k)[keep].copy() for k in range(ctx.ninst)]
+        if timed:
+            ctx.step(warmup, cfl=cfl)
+            ctx.sync()
+            t1 = time.perf_counter()
+            ctx.step(steps, cfl=cfl)          # graph replays; returns after the stream drained
+            wall = time.perf_counter() - t1
+            log(f"{steps} timed steps in {wall:.3f} s")
+            nev = min(steps, 20)
+            tm = ctx.time_steps(nev, cfl=cfl)   # per-kernel event timing, eager launches
+            tb, dtb, fb = ctx.status()
+            rec |= {"value": inf.Nint * steps / wall, "ms_per_step": 1e3 * wall / steps,
+                    "kernel_ms": ({"fused_step": tm["pass_a_ms"]} if sched["fused"] else
+                                  {"pass_a": tm["pass_a_ms"], "pass_b": tm["pass_b_ms"]}) |
+                                 {"dt_reduce": tm["dt_reduce_ms"]},
+                    "faulted_instances": int(fb.sum()),
+                    "steps_total": err_steps + warmup + steps + nev}
+            rec.setdefault("last_dt", dtb)
+        return ctx, rec, states, U0
 
-    ctx, rec, U0 = one(prec)
+    keep_all = slice(0, 4) if (compare_fp64 and prec == L.MIXED) else None
+    ctx, rec, Um, U0 = one(prec, keep=keep_all)
     a_b, b_b, step_b = alg_bytes(rec["Nc"], rec["Nf"], prec == L.MIXED)
     if rec["schedule"]["fused"]:
         kern = {"fused_step": (rec["kernel_ms"]["fused_step"], step_b)}
@@ -428,22 +555,28 @@ def run_batch(name: str, steps: int, warmup: int, tune: bool, check: int, compar
         kern = {"pass_a": (rec["kernel_ms"]["pass_a"], a_b), "pass_b": (rec["kernel_ms"]["pass_b"], b_b)}
     dom = max(kern, key=lambda k: kern[k][0])
     achieved = kern[dom][1] / (kern[dom][0] * 1e-3) / 1e9
+    traffic = read_traffic(name)
     line = {
         "metric": METRIC, "value": rec["value"], "unit": "cell-updates/s", "n_gpus": 1, "steps": steps,
         "warmup": warmup, "ms_per_step": rec["ms_per_step"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32+f64" if prec == L.MIXED else "f64", "data": "synthetic",
         "config": {"workload": f"{name}: {w['n']} wedges, Mach U{list(w['mach'])}, angle U{list(w['angle'])} deg, "
-                               f"log-U{[int(x) for x in w['tris']]} triangles (seed {w['seed']}), one launch per pass, "
-                               f"hipGraph replay, {'autotuned' if tune else 'default'} schedule, CFL {cfl}",
+                               f"log-U{[int(x) for x in w['tris']]} triangles (seed {w['seed']}), "
+                               f"{'FP32-mixed (FP32 differences, gradients, limiter and flux; compensated residual; FP64 update and dt)' if prec == L.MIXED else 'FP64'}, "
+                               f"one launch per step for all instances, hipGraph replay, "
+                               f"{'autotuned' if tune else 'default'} schedule, CFL {cfl}; timed steps follow "
+                               f"{err_steps} untimed steps from the initial state",
                    "instances": w["n"], "generate_s": round(gen_s, 1), **{k: rec[k] for k in
                    ("Nc", "Nint", "Nf", "setup_s", "schedule", "device_bytes")}},
         "hbm_pct_step": 100.0 * step_b / (rec["ms_per_step"] * 1e-3) / (HBM_PEAK_GBS * 1e9),
+        "alg_bytes_per_step": step_b,
         "kernel_ms": rec["kernel_ms"],
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "alg_bytes": kern[dom][1], "traffic": None,
-                     "timing": "event pairs around eager launches right after the graph-timed region"},
-        "faulted_instances": rec["faulted_instances"], "faulted": rec["faulted"], "t_range_ok": rec["t_range"],
-        "cpu_baseline": None,
+                     "frac": achieved / HBM_PEAK_GBS, "alg_bytes": kern[dom][1],
+                     "traffic": (traffic or {}).get(dom),
+                     "timing": "HIP event pairs around eager launches right after the timed region"},
+        "faulted_instances": rec["faulted_instances"],
+        "err_run": rec.get("err_run"),
     }
     if check:
         # the smallest instances against the oracle, from the same initial state, same step count
@@ -462,48 +595,68 @@ def run_batch(name: str, steps: int, warmup: int, tune: bool, check: int, compar
                           "bitwise_equal": bool(np.array_equal(Ug, Uo)),
                           "max_rel": float(np.max(np.abs(Ug - Uo)) / max(np.max(np.abs(Uo)), 1e-300))})
         line["cpu_agreement"] = agree
+    last_dt = rec["last_dt"]
+    ctx.close()
     if compare_fp64 and prec == L.MIXED:
-        Um = [ctx.inst_state(k) for k in range(ctx.ninst)]
-        ctx.close()
-        ctx64, rec64, _ = one(L.FP64)
-        e = []
+        ctx64, rec64, S64, _ = one(L.FP64, timed=True, keep=slice(0, 4))
+        names = ("rho", "rho_u", "rho_v", "rho_E")
+        rel = np.full((4, ctx64.ninst), np.nan)
+        mabs = np.zeros(4)
         Ua = []
         for k in range(ctx64.ninst):
-            Ua.append(ctx64.inst_state(k)[0])
-            e.append(float(np.linalg.norm(Um[k][0] - Ua[k]) / np.linalg.norm(Ua[k])))
+            Ua_k = S64[k]
+            S64[k] = None
+            Ua.append(Ua_k[0].copy())
+            d = Um[k] - Ua_k
+            rel[:, k] = np.linalg.norm(d, axis=1) / np.linalg.norm(Ua_k, axis=1)
+            fin = np.isfinite(d).all(axis=1)
+            mabs = np.where(fin, np.maximum(mabs, np.nanmax(np.abs(d), axis=1)), mabs)
+        sched64 = rec64["schedule"]
         ctx64.close()
         del Um
-        e = np.array(e)
-        # the FP64 path's own sensitivity: the same run from initial states
-        # perturbed by 1e-7 (relative), i.e. about FP32 rounding once, at t = 0
-        ctxp, _, _ = one(L.FP64, perturb=1e-7, sched=rec64["schedule"])
-        fl = np.array([float(np.linalg.norm(ctxp.inst_state(k)[0] - Ua[k]) / np.linalg.norm(Ua[k]))
-                       for k in range(ctxp.ninst)])
-        ctxp.close()
-        line["fp64_batch"] = {"value": rec64["value"], "ms_per_step": rec64["ms_per_step"],
-                              "faulted_instances": rec64["faulted_instances"]}
-        line["speedup_vs_fp64_batch"] = rec["value"] / rec64["value"]
+        e = rel[0]
         fin = np.isfinite(e)
-        line["rho_rel_l2_vs_fp64"] = {"max": float(e[fin].max()) if fin.any() else None,
-                                      "median": float(np.median(e[fin])) if fin.any() else None,
+        line["fp64_batch"] = {"value": rec64["value"], "ms_per_step": rec64["ms_per_step"],
+                              "kernel_ms": rec64["kernel_ms"], "schedule": sched64,
+                              "faulted_instances": rec64["faulted_instances"], "err_run": rec64.get("err_run")}
+        line["speedup_vs_fp64_batch"] = rec["value"] / rec64["value"]
+        line["error_vs_fp64"] = {
+            "what": f"per instance, relative L2 of each conserved variable of the FP32-mixed batch against the "
+                    f"FP64 batch after the same {err_steps} CFL steps from the same initial state; max_abs over all "
+                    f"cells of the instances that stayed finite",
+            **{names[v]: {"rel_l2_median": float(np.median(rel[v][np.isfinite(rel[v])])) if fin.any() else None,
+                          "rel_l2_max": float(np.max(rel[v][np.isfinite(rel[v])])) if fin.any() else None,
+                          "max_abs": float(mabs[v])} for v in range(4)}}
+        line["rho_rel_l2_vs_fp64"] = {"median": float(np.median(e[fin])) if fin.any() else None,
+                                      "max": float(e[fin].max()) if fin.any() else None,
                                       "non_finite_instances": int((~fin).sum()),
                                       "per_instance": [float(x) for x in e]}
-        ok = fin & np.isfinite(fl)
-        line["fp64_sensitivity"] = {
-            "what": "rho rel L2 of the FP64 batch run from initial states perturbed by 1e-7 (relative, seeded) "
-                    "against the unperturbed FP64 run, same steps",
-            "median": float(np.median(fl[ok])) if ok.any() else None,
-            "max": float(fl[ok].max()) if ok.any() else None,
-            "per_instance": [float(x) for x in fl]}
-        line["rho_error_bound"] = {
-            "bound": "rho rel L2 <= 1e-5",
-            "instances_within": int((e[fin] <= 1e-5).sum()), "finite_instances": int(fin.sum()),
-            "within_among_fp64_insensitive": int((e[ok & (fl <= 1e-6)] <= 1e-5).sum()),
-            "fp64_insensitive_instances": int((ok & (fl <= 1e-6)).sum()),
-            "median_ratio_to_fp64_sensitivity": float(np.median(e[ok] / np.maximum(fl[ok], 1e-300)))
-            if ok.any() else None}
+        line["rho_error_bound"] = {"bound": "rho rel L2 <= 1e-5 per instance",
+                                   "instances_within": int((e[fin] <= 1e-5).sum()),
+                                   "finite_instances": int(fin.sum()),
+                                   "median_within": bool(fin.any() and np.median(e[fin]) <= 1e-5)}
+        if sensitivity:
+            # the FP64 path's own sensitivity: the same run from initial states
+            # perturbed by 1e-7 (relative), about one FP32 rounding, once at t = 0
+            ctxp, _, Sp, _ = one(L.FP64, perturb=1e-7, sched=sched64, timed=False, keep=0)
+            fl = np.array([float(np.linalg.norm(Sp[k] - Ua[k]) / np.linalg.norm(Ua[k])) for k in range(ctxp.ninst)])
+            ctxp.close()
+            ok = fin & np.isfinite(fl)
+            line["fp64_sensitivity"] = {
+                "what": "rho rel L2 of the FP64 batch run from initial states perturbed by 1e-7 (relative, seeded) "
+                        "against the unperturbed FP64 run, same steps",
+                "median": float(np.median(fl[ok])) if ok.any() else None,
+                "max": float(fl[ok].max()) if ok.any() else None,
+                "instances_above_1e-5": int((fl[ok] > 1e-5).sum()),
+                "per_instance": [float(x) for x in fl]}
+    if cpu_s > 0:
+        line["cpu_baseline"] = batch_cpu_baseline(cases, specs, None, last_dt, cpu_s, 8)
+        if line["cpu_baseline"]:
+            line["gpu_over_cpu"] = line["value"] / line["cpu_baseline"]["value"]
     else:
-        ctx.close()
+        line["cpu_baseline"] = None
+    if secondary:
+        line["secondary"] = secondary_lines()
     return line
 
 
@@ -512,9 +665,14 @@ def main(argv=None) -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=0, help="timed steps (default: the workload's)")
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c3mixed", "c4", "c5", "c2rk", "c2out"])
+    ap.add_argument("--workload", default="c5", choices=["c2", "c3", "c3mixed", "c4", "c5", "c2rk", "c2out"],
+                    help="c5 (default): the 1,024-wedge FP32-mixed batch against the FP64 batch")
     ap.add_argument("--check", type=int, default=0, help="batches: smallest instances checked against the oracle")
     ap.add_argument("--no-compare", action="store_true", help="c5: skip the FP64 batched comparison run")
+    ap.add_argument("--err-steps", type=int, default=None,
+                    help="batches: untimed steps from the initial state before the timed ones, after which the "
+                         "mixed and FP64 batches are compared (default: the workload's quoted steps)")
+    ap.add_argument("--no-sensitivity", action="store_true", help="c5: skip the perturbed FP64 run")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline sample length (0 = skip)")
     ap.add_argument("--graph", action="store_true", help="single meshes: replay a captured step graph in the "
                     "timed region (measured no faster than eager launches on C2)")
@@ -522,6 +680,9 @@ def main(argv=None) -> int:
                     help="interior cell order: Hilbert curve of centroids (sfc) or reverse Cuthill-McKee (rcm)")
     ap.add_argument("--tune", action="store_true", help="run the autotuner before timing")
     ap.add_argument("--no-tune", action="store_true", help="batches: keep the default schedule")
+    ap.add_argument("--aosoa-width", type=int, default=0, choices=[0, 1, 8, 64],
+                    help="batches: the default schedule with this state tile width instead of the autotuner's choice "
+                         "(the profiles' commands use the width the autotuner picks, to leave its trial runs out)")
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--step-path", default="default", choices=["default", "single", "two-pass"],
                     help="single meshes: force the single-kernel step or the two passes")
@@ -535,7 +696,8 @@ def main(argv=None) -> int:
     elif a.workload == "c2out":
         line = run_output_overlap(a.steps or 500, 50)
     elif a.workload in ("c4", "c5"):
-        line = run_batch(a.workload, a.steps, a.warmup, not a.no_tune, a.check, not a.no_compare)
+        line = run_batch(a.workload, a.steps, a.warmup, not a.no_tune, a.check, not a.no_compare, a.err_steps,
+                         not a.no_sensitivity, a.cpu_seconds, not a.no_secondary, a.aosoa_width)
     else:
         line = run_single(a.workload, a.steps, a.warmup, a.cpu_seconds, a.graph, not a.no_secondary, a.tune,
                           a.order, a.step_path)

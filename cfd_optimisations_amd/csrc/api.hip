@@ -300,8 +300,8 @@ hcfd_ctx* create_ctx(int n, const hcfd_mesh_view* views, int precision, int flag
         const size_t ngeo = (size_t)GEO_PLANES * c->deg_bucket * P;
         c->cgeo32 = dmalloc<float>(ngeo);
         c->iv32 = dmalloc<float>(P);
-        // FP32 geometry from the FP64 planes (same values rounded)
-        HIPCHK(launch_d2f((long long)ngeo, c->m.cgeo, c->cgeo32, c->st));
+        // FP32 geometry from the FP64 planes (same values rounded; normal planes A n)
+        HIPCHK(launch_geo_mixed(c->deg_bucket, P, c->m.cgeo, c->cgeo32, c->st));
         HIPCHK(launch_d2f((long long)P, c->m.iv, c->iv32, c->st));
     }
     const int maxtiles = (c->Nint + 15) / 16 + 1;  // smallest pass-A tile: 64 threads / 4 lanes per cell

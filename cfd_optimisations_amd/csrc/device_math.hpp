@@ -82,6 +82,89 @@ __device__ __forceinline__ void rusanov(T gma, const T UL[NV], const T UR[NV], T
     for (int v = 0; v < NV; v++) F[v] = T(0.5) * madd(-S, UR[v] - UL[v], FL[v] + FR[v]);
 }
 
+// ---------------------------------------------------------------------------
+// FP32-mixed flux in difference form.
+//
+// The mixed path never rounds a cell's state to FP32 as such: every FP32
+// quantity is a difference from the state u of the cell that owns the sum.
+// Rounding the FP64 state each step perturbs the solution by about 6e-8 per
+// step, which the flow carries and amplifies; differences formed in FP64 and
+// then rounded carry a relative error of 6e-8 of the difference instead.
+//
+// For a face with outward unit normal n, face states u + dO (own side) and
+// u + dN (neighbour side), the Rusanov flux (src/Base/Flux.cpp:52-81) less the
+// cell's own physical flux E(u).n is
+//     F - E(u).n = 1/2 (dE(dO) + dE(dN)) - 1/2 S (dN - dO),
+// with dE(d) = E(u + d).n - E(u).n evaluated from d without cancellation and
+// S = max |q_side -+ c_side| over both face states (q normal velocity, c sound
+// speed).  Summed over a closed cell, sum_f A_f E(u).n_f = E(u).(sum_f A_f n_f)
+// vanishes, so the residual is the sum of these differences.
+// ---------------------------------------------------------------------------
+struct FluxBase {
+    float rho, mx, my, e;  // the owning cell's state (FP32)
+    float ux, uy, mu, ep;  // velocity, m.u = |m|^2/rho, e + p
+    float p;
+};
+
+__device__ __forceinline__ FluxBase flux_base(float gma, const double u[NV]) {
+    FluxBase b;
+    b.rho = (float)u[0];
+    b.mx = (float)u[1];
+    b.my = (float)u[2];
+    b.e = (float)u[3];
+    const float ir = recip(b.rho);
+    b.ux = b.mx * ir;
+    b.uy = b.my * ir;
+    b.mu = madd(b.mx, b.ux, b.my * b.uy);
+    b.p = (gma - 1.0f) * madd(-0.5f, b.mu, b.e);
+    b.ep = b.e + b.p;
+    return b;
+}
+
+// dE(d) of one face state u + d (see above) and its normal velocity qs and
+// sound speed cs.  The normal N = (N0, N1) is the face's A n (length times unit
+// normal: the mixed path's geometry planes hold it), so every output is A times
+// its unit-normal value; a2 = |N|^2 = A^2 and q = u.N of the owning cell.
+__device__ __forceinline__ void euler_flux_delta(float gma, const FluxBase& b, float q, const float d[NV], float n0,
+                                                 float n1, float a2, float dF[NV], float& qs, float& cs) {
+    const float irs = recip(b.rho + d[0]);                  // 1 / rho_side
+    const float dmn = madd(d[1], n0, d[2] * n1);            // (m_side - m).N
+    const float dq = madd(-q, d[0], dmn) * irs;             // q_side - q
+    qs = q + dq;
+    // |m_side|^2/rho_side - |m|^2/rho = ((2m + dm).dm - (m.u) drho) / rho_side
+    const float dmu = madd(madd(2.0f, b.mx, d[1]), d[1], madd(madd(2.0f, b.my, d[2]), d[2], -(b.mu * d[0]))) * irs;
+    const float dp = (gma - 1.0f) * madd(-0.5f, dmu, d[3]);  // p_side - p
+    cs = vsqrt(gma * (b.p + dp) * irs * a2);                // A c_side
+    dF[0] = dmn;
+    dF[1] = madd(d[1], qs, madd(b.mx, dq, dp * n0));
+    dF[2] = madd(d[2], qs, madd(b.my, dq, dp * n1));
+    dF[3] = madd(d[3] + dp, qs, b.ep * dq);
+}
+
+// A (F - E(u).n) of one face (see above) with N = A n_out, the outward frame of
+// the owning cell.
+__device__ __forceinline__ void rusanov_delta(float gma, const FluxBase& b, const float dO[NV], const float dN[NV],
+                                              float n0, float n1, float F[NV]) {
+    const float q = madd(b.ux, n0, b.uy * n1);
+    const float a2 = madd(n0, n0, n1 * n1);
+    float FO[NV], FN[NV], qo, co, qn, cn;
+    euler_flux_delta(gma, b, q, dO, n0, n1, a2, FO, qo, co);
+    euler_flux_delta(gma, b, q, dN, n0, n1, a2, FN, qn, cn);
+    const float S = dmax(dmax(fabsf(qo - co), fabsf(qn - cn)), dmax(fabsf(qo + co), fabsf(qn + cn)));
+#pragma unroll
+    for (int v = 0; v < NV; v++) F[v] = 0.5f * madd(-S, dN[v] - dO[v], FO[v] + FN[v]);
+}
+
+// One face's term of the CFL rate, A (|u.n| + c): FP64 from FP64 geometry (the
+// FP64 path), or from the mixed path's FP32 A n with A = |A n|.
+__device__ __forceinline__ double cfl_face(double ux, double uy, double c, double A, double n0, double n1) {
+    return A * (fabs(ux * n0 + uy * n1) + c);
+}
+__device__ __forceinline__ double cfl_face(double ux, double uy, double c, float, float N0, float N1) {
+    const float A = vsqrt(madd(N0, N0, N1 * N1));
+    return fabs(ux * (double)N0 + uy * (double)N1) + c * (double)A;
+}
+
 // Limiter face candidate, src/Base/Limiter.cpp:262-266 (eps = 1e-9, :20):
 // min(1, (Umax-U)/d) if d > eps, min(1, (Umin-U)/d) if d < -eps, else 1.
 // The numerator is selected before the one division, so each face costs one

@@ -77,7 +77,8 @@ __device__ __forceinline__ void load_cell_in(const PassArgs& a, int g, CellIn<T,
 #pragma unroll
     for (int k = 0; k < MAXDEG; k++) {
         c.nbx[k] = cnb_at(a, k, g);
-        c.A[k] = geo<MAXDEG>(gp, P, GEO_A, k, g);
+        // the mixed path's normal planes hold A n (A unused)
+        c.A[k] = sizeof(T) == 8 ? geo<MAXDEG>(gp, P, GEO_A, k, g) : T(0);
         c.n0[k] = geo<MAXDEG>(gp, P, GEO_N0, k, g);
         c.n1[k] = geo<MAXDEG>(gp, P, GEO_N1, k, g);
         c.If[k] = geo<MAXDEG>(gp, P, GEO_IF, k, g);
@@ -93,12 +94,18 @@ __device__ __forceinline__ void load_cell_in(const PassArgs& a, int g, CellIn<T,
 // by nbx leaves every sum exactly as in the two-pass kernels.
 template <typename T, int MAXDEG>
 __device__ __forceinline__ void fused_cell_a(const PassArgs& a, const TileLists& L, const CellIn<T, MAXDEG>& c, int l,
-                                             bool own, const typename FusedTypes<T>::V* __restrict__ sU,
-                                             int su_pitch, typename FusedTypes<T>::V* __restrict__ sFS,
-                                             int fs_pitch, int lnb_out[MAXDEG], bool& bad) {
+                                             bool own, const double2* __restrict__ sU, int su_pitch,
+                                             typename FusedTypes<T>::V* __restrict__ sFS, int fs_pitch,
+                                             int lnb_out[MAXDEG], bool& bad) {
     using FT = FusedTypes<T>;
+    // FP64: the state itself; FP32-mixed: differences from it (u = 0, see
+    // rusanov_delta), formed from the FP64 staged states
+    constexpr bool DELTA = sizeof(T) == 4;
+    double ud[NV];
+    FusedTypes<double>::get(sU, su_pitch, l, ud);
     T u[NV];
-    FT::get(sU, su_pitch, l, u);
+#pragma unroll
+    for (int v = 0; v < NV; v++) u[v] = DELTA ? T(0) : (T)ud[v];
     T g2[NV][2], umin[NV], umax[NV];
 #pragma unroll
     for (int v = 0; v < NV; v++) {
@@ -123,16 +130,19 @@ __device__ __forceinline__ void fused_cell_a(const PassArgs& a, const TileLists&
             q = l;
         }
         ln[k] = q;
+        double und[NV];
+        FusedTypes<double>::get(sU, su_pitch, q, und);
         T un[NV];
-        FT::get(sU, su_pitch, q, un);
+#pragma unroll
+        for (int v = 0; v < NV; v++) un[v] = DELTA ? (T)(und[v] - ud[v]) : (T)und[v];
         const T If = c.If[k];
         const T omI = T(1) - If;
         const bool isR = (c.side >> k) & 1;
         const T wo = isR ? omI : If, wn = isR ? If : omI;
 #pragma unroll
         for (int v = 0; v < NV; v++) {
-            const T uf = madd(u[v], wo, un[v] * wn);
-            const T au = uf * c.A[k];
+            const T uf = DELTA ? un[v] * wn : madd(u[v], wo, un[v] * wn);
+            const T au = DELTA ? uf : uf * c.A[k];  // mixed: n holds A n
             g2[v][0] = madd(au, c.n0[k], g2[v][0]);
             g2[v][1] = madd(au, c.n1[k], g2[v][1]);
             umin[v] = dmin(umin[v], un[v]);
@@ -164,17 +174,19 @@ __device__ __forceinline__ void fused_cell_a(const PassArgs& a, const TileLists&
                 uf[v] = madd(phi[v], d[v], u[v]);
             }
             FT::put(sFS + (size_t)(k * FT::PER) * fs_pitch, fs_pitch, l, uf);
-            if (own && ((ghosts >> k) & 1u)) {
+            // mixed: the ghost's increment is this cell's (pass B uses it directly)
+            if (!DELTA && own && ((ghosts >> k) & 1u)) {
                 // ghost side of a boundary face: U_g + phi_L ((-G_L) . rR), rR = -rL;
                 // the ghost's slot lies in the face-state region [tc + nh1, fs_pitch)
                 if (ln[k] < L.tc + L.nh1 || ln[k] >= fs_pitch) {
                     bad = true;
                     continue;
                 }
-                T ug[NV], ufg[NV];
-                FT::get(sU, su_pitch, ln[k], ug);
+                double ug[NV];
+                T ufg[NV];
+                FusedTypes<double>::get(sU, su_pitch, ln[k], ug);
 #pragma unroll
-                for (int v = 0; v < NV; v++) ufg[v] = madd(phi[v], d[v], ug[v]);
+                for (int v = 0; v < NV; v++) ufg[v] = madd(phi[v], d[v], (T)ug[v]);
                 FT::put(sFS, fs_pitch, ln[k], ufg);
             }
         }
@@ -196,7 +208,7 @@ __device__ __forceinline__ double cfl_bound_in(double gamma, const CellIn<T, MAX
 #pragma unroll
     for (int k = 0; k < MAXDEG; k++) {
         if (ci.nbx[k] == SLOT_EMPTY) continue;
-        lam += (double)ci.A[k] * (fabs(ux * (double)ci.n0[k] + uy * (double)ci.n1[k]) + c);
+        lam += cfl_face(ux, uy, c, ci.A[k], ci.n0[k], ci.n1[k]);
     }
     return cfl_rate((double)ci.iv * lam);
 }
@@ -282,8 +294,10 @@ k_fused_step(
     using V = typename FT::V;
     extern __shared__ double2 fused_lds[];
     const int su_pitch = a.smax, fs_pitch = a.fsmax;
-    V* sU = reinterpret_cast<V*>(fused_lds);
-    V* sFS = sU + (size_t)FT::PER * su_pitch;
+    // staged states are FP64 in both precisions (the mixed path forms its
+    // differences from them); face states (mixed: increments) are T
+    double2* sU = fused_lds;
+    V* sFS = reinterpret_cast<V*>(sU + (size_t)2 * su_pitch);
     TileMin* tmin = reinterpret_cast<TileMin*>(sFS + (size_t)FT::PER * MAXDEG * fs_pitch);
     const int tid = threadIdx.x;
     if (tid == 0) {
@@ -316,7 +330,7 @@ k_fused_step(
         }
         // this lane's tile cell: its inputs are in flight while the tile is staged
         CellIn<T, MAXDEG> ci;
-        double uown[NV] = {0, 0, 0, 0};  // U^n of the lane's tile cell (FP64: read back from LDS in pass B)
+        double uown[NV] = {0, 0, 0, 0};  // U^n of the lane's tile cell (read back from LDS in pass B)
         if (hasT) {
             load_cell_in<T, MAXDEG>(a, i, ci);
             load_local(a.lt, a.NcPad, i, ci);
@@ -345,12 +359,7 @@ k_fused_step(
             load_cell<W, NV>(a.U, gq, uq);
         }
         __syncthreads();  // the previous tile's readers are done
-        if (hasT) {
-            T ut[NV];
-#pragma unroll
-            for (int v = 0; v < NV; v++) ut[v] = (T)uown[v];
-            FT::put(sU, su_pitch, tid, ut);
-        }
+        if (hasT) FusedTypes<double>::put(sU, su_pitch, tid, uown);
         for (int q = tid; q < nhalo; q += FUSED_TILE) {
             int g = gq;
             double ud[NV];
@@ -361,10 +370,7 @@ k_fused_step(
                 g = q < L.nh1 ? a.h1[h0 + q] : a.s2[s0 + q - L.nh1];
                 load_cell<W, NV>(a.U, g, ud);
             }
-            T ut[NV];
-#pragma unroll
-            for (int v = 0; v < NV; v++) ut[v] = (T)ud[v];
-            FT::put(sU, su_pitch, L.tc + q, ut);
+            FusedTypes<double>::put(sU, su_pitch, L.tc + q, ud);
         }
         __syncthreads();
         // pass A: the tile cells, then the halo ring (the last lanes take it)
@@ -394,12 +400,17 @@ k_fused_step(
             const double dt = a.dt[inst];
             double R[NV] = {0.0, 0.0, 0.0, 0.0};
             Kahan K[NV];
+            // U^n of the cell back from LDS (not held in registers through pass A)
+            FusedTypes<double>::get(sU, su_pitch, tid, uown);
+            FluxBase fb;
+            if constexpr (sizeof(T) == 4) fb = flux_base((float)a.gamma, uown);
             unsigned gmask = 0;
 #pragma unroll
             for (int k = 0; k < MAXDEG; k++) {
                 const int nbx = ci.nbx[k];
                 if (nbx == SLOT_EMPTY) continue;
                 const int tw = (int)((unsigned)nbx >> SLOT_TWIN_SHIFT);
+                const bool gnb = (nbx & SLOT_NB_MASK) >= a.Nint;
                 int q = lnb[k];
                 if (q < 0 || q >= fs_pitch || tw >= MAXDEG) {
                     bad = true;
@@ -407,19 +418,33 @@ k_fused_step(
                 }
                 T Uo[NV], Un[NV];
                 FT::get(sFS + (size_t)(k * FT::PER) * fs_pitch, fs_pitch, tid, Uo);
-                FT::get(sFS + (size_t)((tw < MAXDEG ? tw : 0) * FT::PER) * fs_pitch, fs_pitch, q, Un);
                 T F[NV];
-                rusanov((T)a.gamma, Uo, Un, ci.n0[k], ci.n1[k], F);
+                if constexpr (sizeof(T) == 4) {
+                    // increments of the two face states (see pass_b_cell_mixed)
+                    if (gnb) {
+#pragma unroll
+                        for (int v = 0; v < NV; v++) Un[v] = Uo[v];
+                    } else {
+                        FT::get(sFS + (size_t)((tw < MAXDEG ? tw : 0) * FT::PER) * fs_pitch, fs_pitch, q, Un);
+                    }
+                    double und[NV];
+                    FusedTypes<double>::get(sU, su_pitch, q, und);
+#pragma unroll
+                    for (int v = 0; v < NV; v++) Un[v] += (float)(und[v] - uown[v]);
+                    rusanov_delta((float)a.gamma, fb, Uo, Un, ci.n0[k], ci.n1[k], F);
+                } else {
+                    FT::get(sFS + (size_t)((tw < MAXDEG ? tw : 0) * FT::PER) * fs_pitch, fs_pitch, q, Un);
+                    rusanov((T)a.gamma, Uo, Un, ci.n0[k], ci.n1[k], F);
+                }
 #pragma unroll
                 for (int v = 0; v < NV; v++) {
                     if constexpr (sizeof(T) == 8) R[v] = R[v] - ci.A[k] * F[v];
-                    else K[v].add(-(ci.A[k] * F[v]));
+                    else K[v].add(-F[v]);  // rusanov_delta returns A (F - E(u).n)
                 }
-                if ((nbx & SLOT_NB_MASK) >= a.Nint) gmask |= 1u << k;
+                if (gnb) gmask |= 1u << k;
             }
             double unew[NV];
             if constexpr (sizeof(T) == 8) {
-                FT::get(sU, su_pitch, tid, uown);
 #pragma unroll
                 for (int v = 0; v < NV; v++) {
                     R[v] = R[v] * (double)ci.iv;

@@ -1,6 +1,7 @@
 // fused_mixed.hip -- FP32-mixed launchers of the single-kernel step and the CFL
-// bound kernel (FP32 state in LDS, gradient, limiter, face states and flux;
-// compensated residual; FP64 update and CFL bound).
+// bound kernel (FP64 states in LDS; FP32 gradient, limiter, face-state
+// increments and flux in difference form; compensated residual; FP64 update
+// and CFL bound).
 #include "dispatch.hpp"
 #include "fused_impl.hpp"
 

@@ -81,11 +81,11 @@ void build_halo_tables(int Nint, int NcPad, int nslot, const int* cnb, int TC, b
 void free_halo_tables(HaloTables& t);
 
 // Bytes of dynamic LDS the single-kernel step needs (block of 256 lanes):
-// staged states and face states (FP64: 32 B each, FP32-mixed: 16 B) and the
-// tile's CFL slots (24 B, in 32).
+// staged states (FP64, 32 B, in both precisions), face states (FP64: 32 B,
+// FP32-mixed increments: 16 B) and the tile's CFL slots (24 B, in 32).
 inline size_t fused_lds_bytes(bool mixed, int maxdeg, int smax, int fsmax) {
     const size_t per = mixed ? 16 : 32;
-    return per * (size_t)smax + per * (size_t)maxdeg * fsmax + 32;
+    return 32 * (size_t)smax + per * (size_t)maxdeg * fsmax + 32;
 }
 
 }  // namespace hcfd

@@ -494,6 +494,19 @@ __global__ void k_permute_d(int n, const int* __restrict__ old_of_new, const dou
     dst[i] = src[old_of_new[i]];
 }
 
+// FP32 per-slot geometry of the mixed path: the planes of k_pack_cellgeo
+// rounded to FP32, except that the normal planes hold A n (face length times the
+// outward unit normal, the product taken in FP64 and rounded once).
+__global__ void k_geo_mixed(int nslot, long long P, const double* __restrict__ g, float* __restrict__ g32) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long long)nslot * P) return;
+    const long long a = (long long)GEO_A * nslot * P + e;
+    for (int p = 0; p < GEO_PLANES; p++) {
+        const long long q = (long long)p * nslot * P + e;
+        g32[q] = (p == GEO_N0 || p == GEO_N1) ? (float)(g[a] * g[q]) : (float)g[q];
+    }
+}
+
 __global__ void k_d2f(long long n, const double* __restrict__ src, float* __restrict__ dst) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -972,6 +985,12 @@ void build_device_mesh(const BuildInput& in, DeviceMesh& out, hipStream_t st) {
         out.bandwidth_out = std::max(out.bandwidth_out, std::abs(out.new_of_old[a] - out.new_of_old[b]));
     }
     HIPCHK(hipStreamSynchronize(st));
+}
+
+hipError_t launch_geo_mixed(int nslot, long long P, const double* cgeo, float* cgeo32, hipStream_t st) {
+    const long long n = (long long)nslot * P;
+    if (n > 0) k_geo_mixed<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(nslot, P, cgeo, cgeo32);
+    return hipGetLastError();
 }
 
 hipError_t launch_d2f(long long n, const double* src, float* dst, hipStream_t st) {

@@ -59,5 +59,7 @@ void build_device_mesh(const BuildInput& in, DeviceMesh& out, hipStream_t stream
 void free_device_mesh(DeviceMesh& m);
 // dst[i] = (float)src[i], i < n (FP32 geometry and 1/V of the mixed path).
 hipError_t launch_d2f(long long n, const double* src, float* dst, hipStream_t st);
+// FP32 geometry planes of the mixed path from the FP64 ones (normal planes: A n).
+hipError_t launch_geo_mixed(int nslot, long long P, const double* cgeo, float* cgeo32, hipStream_t st);
 
 }  // namespace hcfd

@@ -20,7 +20,7 @@ struct PassArgs {
     const int* cnb = nullptr;         // [MAXDEG][NcPad] neighbour | twin slot << 29, or SLOT_EMPTY
     const int* cside = nullptr;       // [NcPad] side bits (bit k: R side of slot k's face)
     const double* cgeo = nullptr;     // [GEO_PLANES][MAXDEG][NcPad] per-slot geometry, FP64
-    const float* cgeo32 = nullptr;    // same, FP32 (mixed path)
+    const float* cgeo32 = nullptr;    // same, FP32 (mixed path; the normal planes hold A n)
     const double* iv = nullptr;
     const float* iv32 = nullptr;
     const double* U = nullptr;   // current state (interior + ghosts)

@@ -185,9 +185,12 @@ __device__ __forceinline__ double pass_a_cell(const PassArgs& a, int i, const do
     const T* __restrict__ gp = PT::geo(a);
     double ud[NV];
     lds_cell(sU, i - tb, ud);
+    // FP64: the state itself; FP32-mixed: differences from it (u = 0, see
+    // rusanov_delta), so the state is never rounded to FP32
+    constexpr bool DELTA = sizeof(T) == 4;
     T u[NV];
 #pragma unroll
-    for (int v = 0; v < NV; v++) u[v] = (T)ud[v];
+    for (int v = 0; v < NV; v++) u[v] = DELTA ? T(0) : (T)ud[v];
     const T iv = PT::iv(a, i);
     const int side = plane_at(a.cside, i);
     T g[NV][2], umin[NV], umax[NV];
@@ -233,16 +236,18 @@ __device__ __forceinline__ double pass_a_cell(const PassArgs& a, int i, const do
         const T wo = isR ? omI : If, wn = isR ? If : omI;
 #pragma unroll
         for (int v = 0; v < NV; v++) {
-            const T un = (T)und[v];
-            const T uf = madd(u[v], wo, un * wn);
-            const T au = uf * A;
+            // mixed: un = u_nb - u and uf - u = (u_nb - u) wn
+            const T un = DELTA ? (T)(und[v] - ud[v]) : (T)und[v];
+            const T uf = DELTA ? un * wn : madd(u[v], wo, un * wn);
+            // mixed: n0, n1 hold A n
+            const T au = DELTA ? uf : uf * A;
             g[v][0] = madd(au, n0, g[v][0]);
             g[v][1] = madd(au, n1, g[v][1]);
             umin[v] = dmin(umin[v], un);
             umax[v] = dmax(umax[v], un);
         }
         if (nb >= a.Nint) ghosts |= 1u << k;
-        if constexpr (CFL) lam += (double)A * (fabs(ux * (double)n0 + uy * (double)n1) + c);
+        if constexpr (CFL) lam += cfl_face(ux, uy, c, A, n0, n1);
     }
 #pragma unroll
     for (int v = 0; v < NV; v++) {
@@ -274,7 +279,8 @@ __device__ __forceinline__ double pass_a_cell(const PassArgs& a, int i, const do
                 uf[v] = madd(phi[v], d[v], u[v]);
             }
             store_fs(FS, P, k, i, uf);
-            if ((ghosts >> k) & 1u) {
+            // mixed: the ghost's increment is this cell's (pass B uses it directly)
+            if (!DELTA && ((ghosts >> k) & 1u)) {
                 // the ghost side (R) of a boundary face: U_g + phi_L ((-G_L) . rR), rR = -rL
                 const int gc = cnb_at(a, k, i) & SLOT_NB_MASK;
                 double ugd[NV];
@@ -300,7 +306,8 @@ __device__ __forceinline__ double pass_a_cell_quad(const PassArgs& a, int i, int
     const long long P = a.NcPad;
     const T* __restrict__ gp = PT::geo(a);
     const double udv = a.U[cidx<W, NV>(i, v)];
-    const T u = (T)udv;
+    constexpr bool DELTA = sizeof(T) == 4;  // mixed: differences from the state (see pass_a_cell)
+    const T u = DELTA ? T(0) : (T)udv;
     const T iv = PT::iv(a, i);
     const int side = plane_at(a.cside, i);
     T g0 = T(0), g1 = T(0), umin = u, umax = u;
@@ -333,17 +340,18 @@ __device__ __forceinline__ double pass_a_cell_quad(const PassArgs& a, int i, int
         const T If = geo<MAXDEG>(gp, P, GEO_IF, k, i);
         r0[k] = geo<MAXDEG>(gp, P, GEO_R0, k, i);
         r1[k] = geo<MAXDEG>(gp, P, GEO_R1, k, i);
-        const T un = (T)a.U[cidx<W, NV>(nb, v)];
+        const double unb = a.U[cidx<W, NV>(nb, v)];
+        const T un = DELTA ? (T)(unb - udv) : (T)unb;
         const T omI = T(1) - If;
         const bool isR = (side >> k) & 1;
-        const T uf = madd(u, isR ? omI : If, un * (isR ? If : omI));
-        const T au = uf * A;
+        const T uf = DELTA ? un * (isR ? If : omI) : madd(u, isR ? omI : If, un * (isR ? If : omI));
+        const T au = DELTA ? uf : uf * A;  // mixed: n0, n1 hold A n
         g0 = madd(au, n0, g0);
         g1 = madd(au, n1, g1);
         umin = dmin(umin, un);
         umax = dmax(umax, un);
         if (nb >= a.Nint) ghosts |= 1u << k;
-        if constexpr (CFL) lam += (double)A * (fabs(ux * (double)n0 + uy * (double)n1) + c);
+        if constexpr (CFL) lam += cfl_face(ux, uy, c, A, n0, n1);
     }
     g0 *= iv;
     g1 *= iv;
@@ -358,7 +366,7 @@ __device__ __forceinline__ double pass_a_cell_quad(const PassArgs& a, int i, int
         if (k < deg) {
             const T d = dot_r(g0, g1, r0[k], r1[k]);
             FS[fs_index<T>(P, k, i, v)] = madd(phi, d, u);
-            if ((ghosts >> k) & 1u) {
+            if (!DELTA && ((ghosts >> k) & 1u)) {
                 const int gc = cnb_at(a, k, i) & SLOT_NB_MASK;
                 const T ug = (T)a.U[cidx<W, NV>(gc, v)];
                 FS[fs_index<T>(P, 0, gc, v)] = madd(phi, d, ug);
@@ -534,8 +542,9 @@ __device__ __forceinline__ void pass_b_cell_fp64(const PassArgs& a, int i, doubl
 }
 
 // ---------------------------------------------------------------------------
-// Pass B, FP32-mixed: flux in FP32 from FP32 face states, residual summed with
-// Kahan compensation, update and ghost states in FP64.
+// Pass B, FP32-mixed: flux in FP32 in difference form (rusanov_delta) from the
+// FP32 face-state increments and the FP64 neighbour differences, residual
+// summed with Kahan compensation, update and ghost states in FP64.
 // ---------------------------------------------------------------------------
 struct Kahan {
     float s = 0.0f, c = 0.0f;
@@ -556,6 +565,7 @@ __device__ __forceinline__ void pass_b_cell_mixed(const PassArgs& a, int i, doub
     load_cell<W, NV>(a.U, i, u);
     const float iv = a.iv32[i];
     const float gma = (float)a.gamma;
+    const FluxBase fb = flux_base(gma, u);
     Kahan R[NV];
     int gg[MAXDEG];
 #pragma unroll
@@ -565,14 +575,25 @@ __device__ __forceinline__ void pass_b_cell_mixed(const PassArgs& a, int i, doub
         if (nbx == SLOT_EMPTY) continue;
         const int nb = nbx & SLOT_NB_MASK;
         const int tw = (int)((unsigned)nbx >> SLOT_TWIN_SHIFT);
-        float Uo[NV], Un[NV];
-        lds_fs(sF, k, i - tb, Uo);
-        tile_or_global_fs(a.FS32, P, sF, tb, tc, tw, nb, Un);
-        const float A = geo<MAXDEG>(a.cgeo32, P, GEO_A, k, i);
-        float F[NV];
-        rusanov(gma, Uo, Un, geo<MAXDEG>(a.cgeo32, P, GEO_N0, k, i), geo<MAXDEG>(a.cgeo32, P, GEO_N1, k, i), F);
+        // increments of the two face states: own phi (G . r); the neighbour's
+        // u_nb - u plus its own increment (a ghost's increment is this cell's)
+        float dO[NV], dN[NV];
+        lds_fs(sF, k, i - tb, dO);
+        double un[NV];
+        load_cell<W, NV>(a.U, nb, un);
+        if (nb >= a.Nint) {
 #pragma unroll
-        for (int v = 0; v < NV; v++) R[v].add(-(A * F[v]));
+            for (int v = 0; v < NV; v++) dN[v] = dO[v];
+        } else {
+            tile_or_global_fs(a.FS32, P, sF, tb, tc, tw, nb, dN);
+        }
+#pragma unroll
+        for (int v = 0; v < NV; v++) dN[v] += (float)(un[v] - u[v]);
+        float F[NV];  // A (F - E(u).n): the mixed geometry planes hold A n
+        rusanov_delta(gma, fb, dO, dN, geo<MAXDEG>(a.cgeo32, P, GEO_N0, k, i), geo<MAXDEG>(a.cgeo32, P, GEO_N1, k, i),
+                      F);
+#pragma unroll
+        for (int v = 0; v < NV; v++) R[v].add(-F[v]);
         if (nb >= a.Nint) gg[k] = nb;
     }
     double unew[NV];
