@@ -208,7 +208,8 @@ def run_single(name: str, steps: int, warmup: int, cpu_s: float, graph: bool, se
         "dtype": "f32+f64" if prec == L.MIXED else "f64", "data": "synthetic",
         "config": {"workload": desc, "Nc": inf.Nc, "Nint": inf.Nint, "Nf": inf.Nf, "steps": steps, "cfl": cfl,
                    "schedule": ctx.schedule(), "graph": graph, "setup_s": round(setup_s, 2), "cell_order": order,
-                   "rcm_bandwidth": [inf.bandwidth_in, inf.bandwidth_out], "face_colours": inf.ncolours},
+                   "bandwidth": {"what": "max |CL - CR| over interior faces", "caller_order": inf.bandwidth_in,
+                                 f"{'hilbert' if order == 'sfc' else 'rcm'}_order": inf.bandwidth_out}},
         "hbm_pct_step": 100.0 * step_b / (ms_step * 1e-3) / (HBM_PEAK_GBS * 1e9),
         "alg_bytes_per_step": step_b,
         "kernel_ms": ({"fused_step": tm["pass_a_ms"]} if fused else

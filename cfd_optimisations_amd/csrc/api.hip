@@ -110,6 +110,7 @@ struct hcfd_ctx {
     bool fused_default = false;  // ... and is the default schedule (compact tiles)
     size_t fused_lds = 0;
     bool bound_valid = false;  // dt holds the CFL step of the current state (single-kernel path)
+    double bound_cfl = 0.0;    // ... for this CFL number
     double t_host = 0.0;
     double fixed_dt_loaded = NAN;
     // output slots: primitives on the device, their pinned host copy (+ the clock)
@@ -488,12 +489,15 @@ void load_fixed_dt(hcfd_ctx* c, double dt) {
 
 // Single-kernel path, CFL mode: dt must hold the CFL step of the current state
 // before the first step (afterwards every step reduces the bound of its result).
+// A call with another CFL number than the one the stored dt was reduced with
+// takes the bound again, so its first step uses its own CFL number.
 void ensure_bound(hcfd_ctx* c, double cfl) {
-    if (!c->sched.fused || !(cfl > 0) || c->bound_valid) return;
+    if (!c->sched.fused || !(cfl > 0) || (c->bound_valid && c->bound_cfl == cfl)) return;
     Launch L = make_launch(c, true, c->cur, 2);
     HIPCHK(launch_cfl_bound(L.v, L.a, L.grid, c->st));
     HIPCHK(launch_dt_reduce(L.a, cfl, c->dt, c->st));
     c->bound_valid = true;
+    c->bound_cfl = cfl;
 }
 
 // Buffers of one step from U[cur]: stage k reads U[in[k]] and writes U[out[k]];
@@ -574,6 +578,7 @@ int enqueue_step(hcfd_ctx* c, int cur, double cfl, hipEvent_t* ev = nullptr) {
     }
     if (ev) HIPCHK(hipEventRecord(ev[3], c->st));
     c->bound_valid = c->sched.fused && use_cfl;
+    c->bound_cfl = cfl;
     return p.result;
 }
 
@@ -852,6 +857,7 @@ int hcfd_get_permutation(hcfd_ctx* c, int* new_of_old) {
 int hcfd_get_face_colours(hcfd_ctx* c, int* colour_of_face, int* ncolours) {
     return guarded([&] {
         if (!c) fail(HCFD_ERR_ARG, "null");
+        colour_faces(c->m, c->st);  // built on the first request
         if (colour_of_face) std::memcpy(colour_of_face, c->m.colour.data(), sizeof(int) * c->Nf);
         if (ncolours) *ncolours = c->m.ncolours;
     });
@@ -863,6 +869,7 @@ int hcfd_set_bcs(hcfd_ctx* c, int nbc, const int* bctype, const double* qbc, dou
         if (c->ninst != 1) fail(HCFD_ERR_STATE, "batches take their BCs at creation");
         if (!(gamma > 1.0)) fail(HCFD_ERR_ARG, "gamma must exceed 1");
         c->gamma = gamma;
+        c->bound_valid = false;  // the CFL bound depends on gamma
         std::vector<BCEntry> tab(c->nbctab);
         fill_bctab(c, 0, nbc, bctype, qbc, gamma, tab);
         upload_bctab(c, tab);

@@ -690,6 +690,42 @@ __global__ void k_max_reduce(int n, const int* __restrict__ a, int* __restrict__
 
 }  // namespace
 
+// Jones-Plassmann rounds over the face conflict graph (faces sharing a cell),
+// from the cell->face CSR in caller numbering that the build leaves on the host.
+void colour_faces(DeviceMesh& m, hipStream_t st) {
+    if (!m.colour.empty() || m.Nf == 0) return;
+    const int Nc = m.Nc, Nf = m.Nf;
+    // the two cells of every face, from the CSR (order within a face is irrelevant)
+    std::vector<int2> fc(Nf, make_int2(-1, -1));
+    for (int i = 0; i < Nc; i++)
+        for (int k = m.cf_off[i]; k < m.cf_off[i + 1]; k++) {
+            int2& c = fc[m.cf_con[k]];
+            if (c.x < 0) c.x = i; else c.y = i;
+        }
+    Scratch sc;
+    int2* d_fc = sc.get<int2>(Nf);
+    int* d_off = sc.get<int>(Nc + 1);
+    int* d_cf = sc.get<int>(2 * (size_t)Nf);
+    int* d_colour = sc.get<int>(Nf);
+    int* d_n = sc.get<int>(1);
+    h2d(d_fc, fc.data(), Nf, st);
+    h2d(d_off, m.cf_off.data(), Nc + 1, st);
+    h2d(d_cf, m.cf_con.data(), 2 * (size_t)Nf, st);
+    k_fill_int<<<nblk(Nf), 256, 0, st>>>(Nf, d_colour, -1);
+    for (int round = 0;; round++) {
+        if (round > 1000) fail(HCFD_ERR_MESH, "face colouring did not converge");
+        HIPCHK(hipMemsetAsync(d_n, 0, sizeof(int), st));
+        k_colour_round<<<nblk(Nf), 256, 0, st>>>(Nf, d_fc, d_off, d_cf, d_colour, d_n);
+        if (read_int(d_n, st) == 0) break;
+    }
+    std::vector<int> col(Nf);
+    d2h(col.data(), d_colour, Nf, st);
+    int nc = 0;
+    for (int f = 0; f < Nf; f++) nc = std::max(nc, col[f] + 1);
+    m.colour.swap(col);
+    m.ncolours = nc;
+}
+
 void free_device_mesh(DeviceMesh& m) {
     for (void* p : {(void*)m.slots, (void*)m.faces, (void*)m.cnb, (void*)m.cside, (void*)m.cgeo, (void*)m.iv,
                     (void*)m.ghost_bc, (void*)m.inst_int_off, (void*)m.chunk_inst, (void*)m.xf, (void*)m.xc})
@@ -956,23 +992,12 @@ void build_device_mesh(const BuildInput& in, DeviceMesh& out, hipStream_t st) {
         HIPCHK(hipStreamSynchronize(st));
     }
 
-    // ---- face colouring ---------------------------------------------------------------
-    int* d_colour = sc.get<int>(Nf);
-    k_fill_int<<<nblk(Nf), 256, 0, st>>>(Nf, d_colour, -1);
-    for (int round = 0;; round++) {
-        if (round > 1000) fail(HCFD_ERR_MESH, "face colouring did not converge");
-        HIPCHK(hipMemsetAsync(d_n, 0, sizeof(int), st));
-        k_colour_round<<<nblk(Nf), 256, 0, st>>>(Nf, d_fc, d_off, d_cf, d_colour, d_n);
-        if (read_int(d_n, st) == 0) break;
-    }
-
     // ---- host copies of the bookkeeping ------------------------------------------------
+    // (the face colouring is built on request, colour_faces)
     out.face_new_of_old.resize(Nf);
     d2h(out.face_new_of_old.data(), d_face_new_of_old, Nf, st);
-    out.colour.resize(Nf);
-    d2h(out.colour.data(), d_colour, Nf, st);
+    out.colour.clear();
     out.ncolours = 0;
-    for (int f = 0; f < Nf; f++) out.ncolours = std::max(out.ncolours, out.colour[f] + 1);
     out.cf_off.resize(Nc + 1);
     d2h(out.cf_off.data(), d_off, Nc + 1, st);
     out.cf_con.resize(2 * (size_t)Nf);

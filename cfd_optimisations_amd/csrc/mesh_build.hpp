@@ -47,7 +47,7 @@ struct DeviceMesh {
     double* xc = nullptr;           // [2*Nc] caller order
     std::vector<int> new_of_old, old_of_new;     // cells
     std::vector<int> face_new_of_old;            // faces
-    std::vector<int> colour;                     // caller face order
+    std::vector<int> colour;                     // caller face order (empty until colour_faces)
     std::vector<int> cf_off, cf_con;             // caller numbering, ascending face id
     std::vector<int> inst_int_off_h;             // [ninst+1]
     std::vector<int> inst_ghost_count;           // [ninst]
@@ -57,6 +57,9 @@ struct DeviceMesh {
 // Throws hcfd::Error on invalid meshes or HIP failures.
 void build_device_mesh(const BuildInput& in, DeviceMesh& out, hipStream_t stream);
 void free_device_mesh(DeviceMesh& m);
+// Conflict-free face colouring (no two faces of a colour share a cell), caller
+// face order, into m.colour / m.ncolours; built once, on the first request.
+void colour_faces(DeviceMesh& m, hipStream_t stream);
 // dst[i] = (float)src[i], i < n (FP32 geometry and 1/V of the mixed path).
 hipError_t launch_d2f(long long n, const double* src, float* dst, hipStream_t st);
 // FP32 geometry planes of the mixed path from the FP64 ones (normal planes: A n).

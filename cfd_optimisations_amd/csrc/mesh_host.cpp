@@ -2,6 +2,7 @@
 // (the format of src/Mesh.cpp:11-45 and interface/Mesh.py:441-463), the .hsln
 // writer (src/Writer.cpp:22-39) and seeded mesh generators.
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -25,11 +26,30 @@ struct File {
     }
 };
 
+// Bytes left in the file after the current position.
+size_t remaining(FILE* f) {
+    const long pos = ftell(f);
+    if (pos < 0 || fseek(f, 0, SEEK_END) != 0) return 0;
+    const long end = ftell(f);
+    fseek(f, pos, SEEK_SET);
+    return end > pos ? (size_t)(end - pos) : 0;
+}
+
+// Reads n elements; a count the rest of the file cannot hold is reported as a
+// truncated file before anything is allocated for it.
 template <typename T>
 void rd(FILE* f, std::vector<T>& v, size_t n, const char* what, const std::string& path) {
+    if (n > remaining(f) / sizeof(T)) fail(HCFD_ERR_IO, path + ": truncated .hmsh while reading " + what);
     v.resize(n);
     if (n && fread(v.data(), sizeof(T), n, f) != n)
         fail(HCFD_ERR_IO, path + ": truncated .hmsh while reading " + what);
+}
+
+// A CSR offset array: starts at 0 and never decreases.
+void check_offsets(const std::vector<int>& off, const char* what, const std::string& path) {
+    if (off.empty() || off[0] != 0) fail(HCFD_ERR_IO, path + ": " + what + " does not start at 0");
+    for (size_t i = 1; i < off.size(); i++)
+        if (off[i] < off[i - 1]) fail(HCFD_ERR_IO, path + ": " + what + " decreases at entry " + std::to_string(i));
 }
 
 template <typename T>
@@ -234,6 +254,9 @@ int guard(F&& f) {
     } catch (const std::bad_alloc&) {
         set_last_error("host allocation failed");
         return HCFD_ERR_NOMEM;
+    } catch (const std::exception& e) {
+        set_last_error(std::string("unexpected error: ") + e.what());
+        return HCFD_ERR_ARG;
     }
 }
 }  // namespace
@@ -258,8 +281,9 @@ int hcfd_mesh_load(const char* path, hcfd_mesh** out) {
         m->Nf = hdr[5];
         const int npn = std::max(1, hdr[6]);  // src/Mesh.cpp:19
         const int nch = hdr[7];
+        // counts up to INT_MAX - 1, so that Nc + 1 and Ncol + 1 entries are representable
         if (m->dim != 2 || m->Nn <= 0 || m->Ncol <= 0 || m->NBCcol < 0 || m->NBCcol > m->Ncol || m->Nc <= 0 ||
-            m->Nf <= 0 || nch < 0)
+            m->Nf <= 0 || nch < 0 || m->Nc == INT_MAX || m->Ncol == INT_MAX)
             fail(HCFD_ERR_IO, p + ": bad .hmsh header");
         std::vector<char> raw;
         rd(fh.f, raw, nch, "names", p);
@@ -272,15 +296,17 @@ int hcfd_mesh_load(const char* path, hcfd_mesh** out) {
             pos = e + 1;
         }
         rd(fh.f, m->coltag, m->Ncol, "coltag", p);
-        rd(fh.f, m->coloff, m->Ncol + 1, "coloff", p);
+        rd(fh.f, m->coloff, (size_t)m->Ncol + 1, "coloff", p);
         rd(fh.f, m->nodes, 2 * (size_t)m->Nn, "nodes", p);
         rd(fh.f, m->fn_con, 2 * (size_t)m->Nf, "fn_con", p);
         rd(fh.f, m->fc_con, 2 * (size_t)m->Nf, "fc_con", p);
-        rd(fh.f, m->cn_off, m->Nc + 1, "cn_off", p);
-        if (m->cn_off[m->Nc] < 0) fail(HCFD_ERR_IO, p + ": bad cn_off");
+        check_offsets(m->coloff, "coloff", p);
+        if (m->coloff[m->Ncol] != m->Nf) fail(HCFD_ERR_IO, p + ": coloff does not end at Nf");
+        rd(fh.f, m->cn_off, (size_t)m->Nc + 1, "cn_off", p);
+        check_offsets(m->cn_off, "cn_off", p);
         rd(fh.f, m->cn_con, m->cn_off[m->Nc], "cn_con", p);
-        rd(fh.f, m->cc_off, m->Nc + 1, "cc_off", p);
-        if (m->cc_off[m->Nc] < 0) fail(HCFD_ERR_IO, p + ": bad cc_off");
+        rd(fh.f, m->cc_off, (size_t)m->Nc + 1, "cc_off", p);
+        check_offsets(m->cc_off, "cc_off", p);
         rd(fh.f, m->cc_con, m->cc_off[m->Nc], "cc_con", p);
         m->c_ctypes.resize(m->Nc);
         if (fread(m->c_ctypes.data(), sizeof(int), m->Nc, fh.f) != (size_t)m->Nc) m->c_ctypes.assign(m->Nc, 0);
@@ -381,23 +407,28 @@ int hcfd_meshgen_ramp(int nx, int ny, double length, double height, double x_ram
     });
 }
 
-// Writer::write_soln byte layout (src/Writer.cpp:27-35).
-int hcfd_write_hsln(const char* path, int Nc, const double* const Q[4]) {
+// Writer::write_soln byte layout (src/Writer.cpp:27-35), with the caller's
+// variable names (the Writer's varnames, src/Writer.cpp:13); NULL gives the
+// names Euler passes (src/Base/Euler.cpp:18).
+int hcfd_write_hsln(const char* path, int Nc, const double* const Q[4], const char* const varnames[4]) {
     return guard([&] {
         if (!path || !Q || Nc <= 0) fail(HCFD_ERR_ARG, "bad argument");
+        static const char* const dflt[4] = {"Density", "Velocity_X", "Velocity_Y", "Pressure"};
+        const char* const* names = varnames ? varnames : dflt;
+        int nchars = 0;
+        for (int v = 0; v < 4; v++) {
+            if (!names[v]) fail(HCFD_ERR_ARG, "null variable name");
+            nchars += (int)strlen(names[v]) + 1;
+        }
+        for (int v = 0; v < 4; v++)
+            if (!Q[v]) fail(HCFD_ERR_ARG, "null array");
         std::string p(path);
         File fh(fopen(path, "wb"));
         if (!fh.f) fail(HCFD_ERR_IO, p + ": cannot open for writing");
-        static const char* names[4] = {"Density", "Velocity_X", "Velocity_Y", "Pressure"};
-        int nchars = 0;
-        for (auto* n : names) nchars += (int)strlen(n) + 1;
         int hdr[3] = {Nc, 4, nchars};
         wr(fh.f, hdr, 3, p);
-        for (auto* n : names) wr(fh.f, n, strlen(n) + 1, p);
-        for (int v = 0; v < 4; v++) {
-            if (!Q[v]) fail(HCFD_ERR_ARG, "null array");
-            wr(fh.f, Q[v], Nc, p);
-        }
+        for (int v = 0; v < 4; v++) wr(fh.f, names[v], strlen(names[v]) + 1, p);
+        for (int v = 0; v < 4; v++) wr(fh.f, Q[v], Nc, p);
     });
 }
 

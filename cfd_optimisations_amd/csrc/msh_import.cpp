@@ -11,6 +11,7 @@
 //     higher (the ghost is always on the right), face colour = max tag,
 //     faces grouped by colour (interface/Mesh.py:321-378).
 #include <algorithm>
+#include <climits>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -128,11 +129,13 @@ std::unique_ptr<hcfd_mesh> import_msh(const std::string& path) {
         std::istringstream nd(section(all, "Nodes", path));
         long nblocks, nnodes, mintag, maxtag;
         nd >> nblocks >> nnodes >> mintag >> maxtag;
-        xyz.reserve(2 * nnodes);
+        if (!nd || nblocks < 0 || nnodes < 0 || nnodes > INT_MAX / 2) fail(HCFD_ERR_IO, path + ": bad $Nodes header");
+        xyz.reserve(2 * (size_t)nnodes);
         for (long b = 0; b < nblocks; b++) {
             int edim, etag, para;
             long n;
             nd >> edim >> etag >> para >> n;
+            if (!nd || n < 0 || n > nnodes) fail(HCFD_ERR_IO, path + ": bad $Nodes block header");
             if (para) fail(HCFD_ERR_IO, path + ": parametric nodes are not supported");
             std::vector<long> tags(n);
             for (long k = 0; k < n; k++) nd >> tags[k];
@@ -157,10 +160,12 @@ std::unique_ptr<hcfd_mesh> import_msh(const std::string& path) {
         std::istringstream el(section(all, "Elements", path));
         long nblocks, nelem, mintag, maxtag;
         el >> nblocks >> nelem >> mintag >> maxtag;
+        if (!el || nblocks < 0 || nelem < 0 || nelem > INT_MAX) fail(HCFD_ERR_IO, path + ": bad $Elements header");
         for (long b = 0; b < nblocks; b++) {
             int edim, etag, etype;
             long n;
             el >> edim >> etag >> etype >> n;
+            if (!el || n < 0 || n > nelem) fail(HCFD_ERR_IO, path + ": bad $Elements block header");
             if (etype < 1 || etype > 7) fail(HCFD_ERR_IO, path + ": unsupported element type " + std::to_string(etype));
             const int nn = kNodesOfType[etype];
             Block blk;
@@ -280,14 +285,26 @@ std::unique_ptr<hcfd_mesh> import_msh(const std::string& path) {
         m->c_ctypes[r] = c_type[o];
         tag_new[r] = c_tag[o];
     }
+    // Colours by physical tag: 0 for the surface cells, 1..N-1 for the boundary
+    // curves' tags, which must be exactly those values.  (The reference counts
+    // colours by (tag, element type) block and drops the faces of any tag at or
+    // past that count, interface/Mesh.py:362-372; with contiguous tags and one
+    // element type per tag the two agree.)
+    std::vector<int> tags;
+    for (auto& b : merged) tags.push_back(b.ptag);
+    std::sort(tags.begin(), tags.end());
+    tags.erase(std::unique(tags.begin(), tags.end()), tags.end());
+    for (size_t k = 0; k < tags.size(); k++)
+        if (tags[k] != (int)k)
+            fail(HCFD_ERR_IO, path + ": the boundary curves' physical tags must be 1.." +
+                                  std::to_string(tags.size() - 1) + " (tag " + std::to_string(tags[k]) +
+                                  " found where " + std::to_string(k) + " was expected)");
     // faces: one per neighbour pair (i < j in the new numbering)
-    int maxtag = 0;
-    for (int t : c_tag) maxtag = std::max(maxtag, t);
-    const int ncol = (int)merged.size();
+    const int ncol = (int)tags.size();
     struct F {
         int cl, cr, n1, n2, col;
     };
-    std::vector<std::vector<F>> bycol(std::max(ncol, maxtag + 1));
+    std::vector<std::vector<F>> bycol(ncol);
     for (int r = 0; r < Nc; r++) {
         int o = old_of_new[r];
         for (int q : nbr[o]) {
@@ -303,7 +320,7 @@ std::unique_ptr<hcfd_mesh> import_msh(const std::string& path) {
             if (sh.size() < 2) continue;
             F f{r, s, sh[0], sh[1], std::max(tag_new[r], tag_new[s])};
             if (tag_new[r] > tag_new[s]) std::swap(f.cl, f.cr);
-            if (f.col >= (int)bycol.size()) fail(HCFD_ERR_IO, path + ": physical tags must be 1..N-1");
+            if (f.col >= ncol) fail(HCFD_ERR_IO, path + ": face of an unknown colour");
             bycol[f.col].push_back(f);
         }
     }
@@ -352,5 +369,8 @@ extern "C" int hcfd_mesh_import_msh(const char* path, hcfd_mesh** out) {
     } catch (const std::bad_alloc&) {
         hcfd::set_last_error("host allocation failed");
         return HCFD_ERR_NOMEM;
+    } catch (const std::exception& e) {
+        hcfd::set_last_error(std::string("unexpected error: ") + e.what());
+        return HCFD_ERR_IO;
     }
 }

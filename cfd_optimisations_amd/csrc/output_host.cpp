@@ -35,6 +35,9 @@ int guard(F&& f) {
     } catch (const std::bad_alloc&) {
         set_last_error("host allocation failed");
         return HCFD_ERR_NOMEM;
+    } catch (const std::exception& e) {
+        set_last_error(std::string("unexpected error: ") + e.what());
+        return HCFD_ERR_ARG;
     }
 }
 

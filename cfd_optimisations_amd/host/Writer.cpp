@@ -1,5 +1,5 @@
 // Writer.cpp -- Writer::write_soln's file layout (src/Writer.cpp:22-39),
-// written through the library's checked writer (no <format>).
+// written through the library's checked writer (no <format>), names included.
 #include "Writer.hpp"
 
 #include <stdexcept>
@@ -18,6 +18,8 @@ Writer::Writer(int nvars, const std::string fname, const std::string varnames[NV
 void Writer::write_soln() {
     const std::string path = fname + "_" + std::to_string(runcount) + ".hsln";
     const double* const a[4] = {arrs[0], arrs[1], arrs[2], arrs[3]};
-    if (hcfd_write_hsln(path.c_str(), Nc, a) != HCFD_OK) throw std::runtime_error(hcfd_last_error());
+    const char* const names[4] = {varnames[0].c_str(), varnames[1].c_str(), varnames[2].c_str(),
+                                  varnames[3].c_str()};
+    if (hcfd_write_hsln(path.c_str(), Nc, a, names) != HCFD_OK) throw std::runtime_error(hcfd_last_error());
     runcount++;
 }

@@ -124,7 +124,7 @@ int main(int argc, char** argv) {
             for (int v = 0; v < 4; v++) Q[v] = &Qb[(size_t)v * Nc];
             MI355X::check(hcfd_get_prims(domain.ctx(), Q), "hcfd_get_prims");
             const double* const Qc[4] = {Q[0], Q[1], Q[2], Q[3]};
-            MI355X::check(hcfd_write_hsln((out + "_1.hsln").c_str(), Nc, Qc), "hcfd_write_hsln");
+            MI355X::check(hcfd_write_hsln((out + "_1.hsln").c_str(), Nc, Qc, nullptr), "hcfd_write_hsln");
         }
         auto t2 = std::chrono::steady_clock::now();
         const double s = std::chrono::duration<double>(t2 - t1).count();

@@ -119,7 +119,7 @@ def load() -> ctypes.CDLL:
         "hcfd_mesh_free": [P],
         "hcfd_meshgen_riemann": [I, I, D, ctypes.c_ulonglong, I, P],
         "hcfd_meshgen_ramp": [I, I, D, D, D, D, D, ctypes.c_ulonglong, I, P],
-        "hcfd_write_hsln": [ctypes.c_char_p, I, P],
+        "hcfd_write_hsln": [ctypes.c_char_p, I, P, P],
         "hcfd_read_hsln": [ctypes.c_char_p, P, P],
         "hcfd_write_vtu": [ctypes.c_char_p, P, P],
         "hcfd_step_async": [P, I, D, D],
@@ -494,9 +494,16 @@ class Context:
         return t, d, f
 
 
-def write_hsln(path: str, Q: np.ndarray) -> None:
+def write_hsln(path: str, Q: np.ndarray, names: Optional[Sequence[str]] = None) -> None:
+    """Writer::write_soln's file (src/Writer.cpp:22-39); names: the four variable
+    names (default Density, Velocity_X, Velocity_Y, Pressure)."""
     Q = np.ascontiguousarray(Q, dtype=np.float64)
-    check(load().hcfd_write_hsln(path.encode(), Q.shape[1], _rows(Q)))
+    nm = None
+    if names is not None:
+        if len(names) != 4:
+            raise ValueError("four variable names")
+        nm = (ctypes.c_char_p * 4)(*[n.encode() for n in names])
+    check(load().hcfd_write_hsln(path.encode(), Q.shape[1], _rows(Q), nm))
 
 
 def read_hsln(path: str) -> np.ndarray:

@@ -107,7 +107,8 @@ typedef struct hcfd_info {
     int precision, flags;
     int bandwidth_in;    /* max |CL-CR| over interior faces, caller's order */
     int bandwidth_out;   /* same after renumbering */
-    int ncolours;        /* face colours of the conflict-free colouring */
+    int ncolours;        /* face colours of the conflict-free colouring; 0 until
+                            hcfd_get_face_colours has built it (on its first call) */
     long long device_bytes;
     hcfd_schedule sched;
     int fused_available;  /* the halo tiles of this mesh fit in LDS and their local-index tables
@@ -128,7 +129,7 @@ int hcfd_device_count(int* n);
 
 /* Replaces Base::Domain::Domain(Mesh&) (src/Base/Domain.cpp:4-24) and
  * Mesh::compute_cf_con (src/Mesh.cpp:88-117): uploads the mesh, builds the
- * cell->face CSR, the RCM renumbering, face colouring and the face/cell
+ * cell->face CSR, the RCM (or Hilbert) renumbering and the face/cell
  * geometry on the device, and applies Domain::zero_bc_iv
  * (src/Base/Domain.cpp:148-154) as the Euler constructor does. */
 int hcfd_create(const hcfd_mesh_view* mesh, int precision, int flags, hcfd_ctx** out);
@@ -146,7 +147,8 @@ int hcfd_get_cf(hcfd_ctx* ctx, int* cf_off, int* cf_con);
 int hcfd_get_permutation(hcfd_ctx* ctx, int* new_of_old);
 /* Conflict-free face colouring (no two faces of one colour share a cell);
  * colour_of_face in the caller's face order.  Replaces the empty
- * Mesh::colour_faces (src/Mesh.cpp:119-121). */
+ * Mesh::colour_faces (src/Mesh.cpp:119-121).  Built on the device on the first
+ * call (Jones-Plassmann rounds), not at creation: no step kernel needs it. */
 int hcfd_get_face_colours(hcfd_ctx* ctx, int* colour_of_face, int* ncolours);
 
 /* Boundary conditions by BC ordinal: Euler::gather_BCs (src/Base/Euler.cpp:42-60)
@@ -297,8 +299,10 @@ int hcfd_meshgen_ramp(int nx, int ny, double length, double height, double x_ram
                       double jitter, unsigned long long seed, int shuffle, hcfd_mesh** out);
 
 /* Writer::write_soln (src/Writer.cpp:22-39): <fname>_<runcount>.hsln with the
- * primitive variables of all Nc cells. */
-int hcfd_write_hsln(const char* path, int Nc, const double* const Q[4]);
+ * primitive variables of all Nc cells under the Writer's variable names
+ * (src/Writer.cpp:13, 31); varnames NULL gives Density, Velocity_X, Velocity_Y,
+ * Pressure. */
+int hcfd_write_hsln(const char* path, int Nc, const double* const Q[4], const char* const varnames[4]);
 /* The .hsln reader of interface/Export.py:14-32 (read_soln), for restarts:
  * checks nvars == 4 and the file size, sets *Nc and, when Q is non-NULL, fills
  * Q[v][0..Nc).  A restart is hcfd_initialise_prims with the arrays read. */

@@ -178,6 +178,31 @@ def test_fixed_dt_after_cfl_steps():
             assert np.array_equal(ctx.get_state(), U), sched
 
 
+@pytest.mark.parametrize("graph", [0, L.FLAG_GRAPH], ids=["eager", "graph"])
+@pytest.mark.parametrize("sched", [TWO_PASS, FUSED], ids=["two_pass", "fused"])
+def test_cfl_number_changes_between_calls(sched, graph):
+    """Five steps at CFL 0.5, then five at 0.3 (as a caller lowers the CFL number
+    after an instability), then five at 0.5 again: every step takes its dt from
+    its own call's CFL number, the first step of each call included."""
+    case = C.riemann_case(30, 20, seed=5)
+    od, U = oracle_for(case)
+    t, hs = 0.0, []
+    for cfl in (0.5, 0.3, 0.5):
+        for _ in range(5):
+            h = od.cfl_dt(U, cfl)
+            U = od.run(U, 1, dt=h)[0]
+            t += h
+            hs.append(h)
+    with C.make_context(case, L.FP64, L.FLAG_ORDER_SFC | graph) as ctx:
+        ctx.set_schedule(**sched)
+        ctx.step(5, cfl=0.5)
+        ctx.step(5, cfl=0.3)
+        assert ctx.last_dt() == hs[9]
+        assert ctx.step(5, cfl=0.5) == t
+        assert ctx.last_dt() == hs[14]
+        assert np.array_equal(ctx.get_state(), U)
+
+
 @pytest.mark.parametrize("shuffle", [True, False])
 @pytest.mark.parametrize("sched", [TWO_PASS, FUSED], ids=["two_pass", "fused"])
 def test_riemann_cfl_exact(shuffle, sched):
@@ -215,21 +240,34 @@ def test_inverted_cell_rejected():
         L.Context.create(bad, L.FP64, 0)
 
 
-def test_permutation_and_colouring():
-    case = C.riemann_case(30, 20, seed=3, shuffle=True)
+@pytest.mark.parametrize("nx,ny", [(30, 20), (20, 30), (64, 16)])
+def test_permutation_and_colouring(nx, ny):
+    """The device reverse Cuthill-McKee order of a shuffled structured mesh: a
+    permutation with interior cells first whose bandwidth (max |CL - CR| over
+    interior faces) is that of a level structure across the short side of the
+    grid, at most 2 min(nx, ny) + 8 cells (the shuffled order's is about the cell
+    count); a Hilbert order is a permutation too.  The face colouring, built on
+    request, is conflict-free with at most 2 d - 1 colours for cells of d faces
+    (the greedy bound on the conflict graph, whose degree is 2 (d - 1))."""
+    case = C.riemann_case(nx, ny, seed=3, shuffle=True)
     hm = case.hmesh()
-    with C.make_context(case) as ctx:
-        inf = ctx.info()
-        p = ctx.permutation()
-        assert np.array_equal(np.sort(p), np.arange(hm.Nc))
-        ghost = hm.ghost_mask()
-        assert (p[ghost] >= inf.Nint).all() and (p[~ghost] < inf.Nint).all()
-        assert inf.bandwidth_out < inf.bandwidth_in
-        col, ncol = ctx.face_colours()
-        assert ncol == col.max() + 1 and col.min() >= 0
-        for c in range(ncol):
-            cells = hm.fc_con[col == c].ravel()
-            assert len(np.unique(cells)) == len(cells), f"colour {c} has two faces sharing a cell"
+    ghost = hm.ghost_mask()
+    for flags in (0, L.FLAG_ORDER_SFC):
+        with C.make_context(case, L.FP64, flags) as ctx:
+            inf = ctx.info()
+            p = ctx.permutation()
+            assert np.array_equal(np.sort(p), np.arange(hm.Nc))
+            assert (p[ghost] >= inf.Nint).all() and (p[~ghost] < inf.Nint).all()
+            assert inf.bandwidth_in > hm.Nc // 2
+            if flags == 0:
+                assert inf.bandwidth_out <= 2 * min(nx, ny) + 8, (inf.bandwidth_out, nx, ny)
+                assert inf.ncolours == 0   # not built until requested
+                col, ncol = ctx.face_colours()
+                assert ctx.info().ncolours == ncol
+                assert ncol == col.max() + 1 and col.min() >= 0 and ncol <= 2 * 3 - 1
+                for c in range(ncol):
+                    cells = hm.fc_con[col == c].ravel()
+                    assert len(np.unique(cells)) == len(cells), f"colour {c} has two faces sharing a cell"
 
 
 def test_schedules_bitwise_identical():
@@ -424,6 +462,41 @@ def test_c2_benchmark_mesh_exact():
         t = ctx.step(2, cfl=w["cfl"])
         assert t == to
         assert np.array_equal(ctx.get_state(), Uo)
+
+
+@pytest.mark.parametrize("name,steps", [("c2", 450), ("c3", 300)])
+def test_developed_flow_full_size_exact(name, steps):
+    """The quoted meshes at full size (4M-triangle Riemann problem, 16M-triangle
+    ramp) in developed flow: from the GPU's own state after `steps` CFL steps, the
+    next step on the GPU equals one oracle step from the same state, bit for bit,
+    dt included.  By then shocks and contacts have formed: the limiter is active
+    (phi < 1) on thousands of cells in every variable."""
+    if name == "c2":
+        w = C.WORKLOADS["c2_riemann_fp64"]
+        case = C.riemann_case(w["nx"], w["ny"], seed=0)
+    else:
+        w = C.WORKLOADS["c3_ramp_16m"]
+        case = C.ramp_case(w["nx"], w["ny"], w["mach"], w["angle"], seed=0)
+    cfl = w["cfl"]
+    with C.make_context(case, L.FP64, L.FLAG_ORDER_SFC) as ctx:
+        assert ctx.schedule()["fused"] == 1
+        ctx.step(steps, cfl=cfl)
+        assert ctx.fault() == 0
+        U = ctx.get_state()
+        ctx.step(1, cfl=cfl)
+        Ug, dtg = ctx.get_state(), ctx.last_dt()
+    hm = case.hmesh()
+    od = ol.OracleDomain(hm, case.gamma, case.bctypes, case.qbc)
+    od.zero_bc_iv()
+    dto = od.cfl_dt(U, cfl)
+    Uo, _ = od.run(U, 1, dt=dto)
+    assert dtg == dto
+    assert np.array_equal(Ug, Uo)
+    inter = ~hm.ghost_mask()
+    Gr = od.bc_grads(od.gradient(U))
+    phi = od.limiter(U, Gr)
+    active = (phi[:, inter] < 1.0).sum(axis=1)
+    assert (active > 1000).all(), active
 
 
 def test_mixed_quads_wedge(wedge_mesh):

@@ -151,35 +151,45 @@ def test_renumbering_preserves_the_solution():
     np.testing.assert_allclose(Ur[:, perm], Ua, rtol=1e-13, atol=1e-13)
 
 
-def write_square_msh(path, nx, ny):
-    """gmsh v4.1 ASCII file of [0,1]^2: nx*ny quads split into triangles, physical
-    curves bottom/right/top/left (tags 1-4) and the surface (tag 5)."""
+def write_square_msh(path, nx, ny, quads_left=0, tags=(1, 2, 3, 4)):
+    """gmsh v4.1 ASCII file of [0,1]^2 on an nx*ny grid: the first quads_left columns
+    are quadrilaterals, the rest quads split into triangles (two surface element
+    blocks of different types when both occur); physical curves bottom/right/top/
+    left with physical tags `tags`, and the surface (tag 5)."""
     NX = nx + 1
     nid = lambda i, j: j * NX + i + 1
-    tris = []
+    tris, quads = [], []
     for j in range(ny):
         for i in range(nx):
-            tris.append((nid(i, j), nid(i + 1, j), nid(i + 1, j + 1)))
-            tris.append((nid(i, j), nid(i + 1, j + 1), nid(i, j + 1)))
+            if i < quads_left:
+                quads.append((nid(i, j), nid(i + 1, j), nid(i + 1, j + 1), nid(i, j + 1)))
+            else:
+                tris.append((nid(i, j), nid(i + 1, j), nid(i + 1, j + 1)))
+                tris.append((nid(i, j), nid(i + 1, j + 1), nid(i, j + 1)))
     sides = [[(nid(i, 0), nid(i + 1, 0)) for i in range(nx)], [(nid(nx, j), nid(nx, j + 1)) for j in range(ny)],
              [(nid(i + 1, ny), nid(i, ny)) for i in range(nx)], [(nid(0, j + 1), nid(0, j)) for j in range(ny)]]
+    t = tags
     L = ["$MeshFormat", "4.1 0 8", "$EndMeshFormat", "$PhysicalNames", "5",
-         '1 1 "bottom"', '1 2 "right"', '1 3 "top"', '1 4 "left"', '2 5 "fluid"', "$EndPhysicalNames",
-         "$Entities", "4 4 1 0",
+         f'1 {t[0]} "bottom"', f'1 {t[1]} "right"', f'1 {t[2]} "top"', f'1 {t[3]} "left"', '2 5 "fluid"',
+         "$EndPhysicalNames", "$Entities", "4 4 1 0",
          "1 0 0 0 0", "2 1 0 0 0", "3 1 1 0 0", "4 0 1 0 0",
-         "1 0 0 0 1 0 0 1 1 2 1 -2", "2 1 0 0 1 1 0 1 2 2 2 -3", "3 0 1 0 1 1 0 1 3 2 3 -4",
-         "4 0 0 0 0 1 0 1 4 2 4 -1", "1 0 0 0 1 1 0 1 5 4 1 2 3 4", "$EndEntities"]
+         f"1 0 0 0 1 0 0 1 {t[0]} 2 1 -2", f"2 1 0 0 1 1 0 1 {t[1]} 2 2 -3", f"3 0 1 0 1 1 0 1 {t[2]} 2 3 -4",
+         f"4 0 0 0 0 1 0 1 {t[3]} 2 4 -1", "1 0 0 0 1 1 0 1 5 4 1 2 3 4", "$EndEntities"]
     nn = NX * (ny + 1)
     L += ["$Nodes", f"1 {nn} 1 {nn}", f"2 1 0 {nn}"] + [str(k + 1) for k in range(nn)]
     L += [f"{(k % NX) / nx!r} {(k // NX) / ny!r} 0" for k in range(nn)] + ["$EndNodes"]
-    ne = len(tris) + 2 * (nx + ny)
-    L += ["$Elements", f"5 {ne} 1 {ne}", f"2 1 2 {len(tris)}"]
+    surf = [(2, tris), (3, quads)]
+    surf = [(ty, el) for ty, el in surf if el]
+    ne = len(tris) + len(quads) + 2 * (nx + ny)
+    L += ["$Elements", f"{len(surf) + 4} {ne} 1 {ne}"]
     tag = 1
-    for t in tris:
-        L.append(f"{tag} {t[0]} {t[1]} {t[2]}")
-        tag += 1
-    for s, edges in enumerate(sides):
-        L.append(f"1 {s + 1} 1 {len(edges)}")
+    for ty, el in surf:
+        L.append(f"2 1 {ty} {len(el)}")
+        for e in el:
+            L.append(f"{tag} " + " ".join(map(str, e)))
+            tag += 1
+    for sd, edges in enumerate(sides):
+        L.append(f"1 {sd + 1} 1 {len(edges)}")
         for a, b in edges:
             L.append(f"{tag} {a} {b}")
             tag += 1
@@ -218,6 +228,73 @@ def test_msh_import_errors(tmp_path):
     p.write_text("$MeshFormat\n2.2 0 8\n$EndMeshFormat\n")
     with pytest.raises(L.HcfdError):
         L.NativeMesh.import_msh(str(p))
+    # negative node and element counts are errors, not allocations
+    good = str(tmp_path / "g.msh")
+    write_square_msh(good, 4, 3)
+    text = open(good).read()
+    for old, new, what in (("$Nodes\n1 20 1 20", "$Nodes\n1 -20 1 20", "Nodes"),
+                           ("2 1 0 20\n", "2 1 0 -20\n", "Nodes"),
+                           ("$Elements\n5 38 1 38", "$Elements\n5 -38 1 38", "Elements")):
+        assert old in text
+        q = tmp_path / "neg.msh"
+        q.write_text(text.replace(old, new))
+        with pytest.raises(L.HcfdError, match=what):
+            L.NativeMesh.import_msh(str(q))
+
+
+def test_msh_import_mixed_cells_and_tags(tmp_path):
+    """A surface of quadrilaterals and triangles (two element blocks with the same
+    physical tag) gives one colour per physical tag; boundary tags that are not
+    exactly 1..N-1 are refused instead of their faces being dropped."""
+    nx, ny, nq = 8, 5, 3
+    p = str(tmp_path / "mixed.msh")
+    write_square_msh(p, nx, ny, quads_left=nq)
+    m = L.NativeMesh.import_msh(p).to_hmesh()
+    ncells = nq * ny + 2 * (nx - nq) * ny
+    assert m.Nc == ncells + 2 * (nx + ny)
+    assert m.Ncol == 5 and m.NBCcol == 4
+    assert list(np.diff(m.coloff))[1:] == [nx, ny, nx, ny]
+    nodes_per_cell = np.diff(m.cn_off)
+    assert (nodes_per_cell == 4).sum() == nq * ny and (nodes_per_cell == 3).sum() == 2 * (nx - nq) * ny
+    ghost = m.ghost_mask()
+    od = ol.OracleDomain(m)
+    assert abs((1 / od.geometry()["iv"][~ghost]).sum() - 1.0) < 1e-14
+    # no face was dropped: each surface cell has as many faces as corners
+    assert m.Nf - m.F_BC == 2 * (nx + ny)
+    cells_per_face = np.bincount(m.fc_con[: m.F_BC].ravel(), minlength=m.Nc)
+    assert (cells_per_face[~ghost] + np.bincount(m.fc_con[m.F_BC:, 0], minlength=m.Nc)[~ghost]
+            == nodes_per_cell[~ghost]).all()
+    for bad in ((1, 2, 3, 5), (2, 3, 4, 5)):
+        q = str(tmp_path / "gap.msh")
+        write_square_msh(q, nx, ny, tags=bad)
+        with pytest.raises(L.HcfdError, match="physical tags"):
+            L.NativeMesh.import_msh(q)
+
+
+def test_hmsh_reader_rejects_bad_counts(tmp_path, wedge_path):
+    """Header counts larger than the file, negative or at INT_MAX, and offset
+    arrays that decrease are I/O errors before anything is allocated for them."""
+    raw = bytearray(open(wedge_path, "rb").read())
+    hdr = np.frombuffer(bytes(raw[:32]), np.int32).copy()
+
+    def attempt(data, name):
+        q = tmp_path / name
+        q.write_bytes(bytes(data))
+        with pytest.raises(L.HcfdError):
+            L.NativeMesh.load(str(q))
+
+    for field, value in ((4, 2**31 - 1), (4, 2**30), (1, 2**30), (5, -3), (7, -1), (2, 2**31 - 1)):
+        h = hdr.copy()
+        h[field] = value
+        attempt(h.tobytes() + bytes(raw[32:]), f"h{field}.hmsh")
+    attempt(raw[: len(raw) // 2], "cut.hmsh")
+    # cn_off made to decrease at cell 5
+    m = L.NativeMesh.load(wedge_path).to_hmesh()
+    off = 32 + int(hdr[7]) + 4 * m.Ncol + 4 * (m.Ncol + 1) + 8 * 2 * m.Nn + 4 * 2 * m.Nf * 2
+    bad = bytearray(raw)
+    bad[off + 4 * 5: off + 4 * 6] = np.int32(-7).tobytes()
+    attempt(bad, "dec.hmsh")
+    assert L.NativeMesh.load(wedge_path).to_hmesh().Nc == m.Nc
 
 
 # ---------------------------------------------------------------------------
@@ -243,6 +320,25 @@ def test_hsln_reader_matches_golden_and_round_trips(tmp_path):
     open(tmp_path / "bad.hsln", "wb").write(bytes(bad))
     with pytest.raises(L.HcfdError, match="nvars"):
         L.read_hsln(str(tmp_path / "bad.hsln"))
+
+
+def test_hsln_writer_takes_the_callers_names(tmp_path):
+    """Writer::write_soln writes whatever names its Writer was given
+    (src/Writer.cpp:13, 27-35): header Nc, nvars, total name bytes; the names with
+    their NUL; then the arrays.  The default is Euler's (src/Base/Euler.cpp:18)."""
+    import struct
+    Q = np.random.default_rng(9).standard_normal((4, 37))
+    names = ["rho", "u_velocity", "v", "p_static"]
+    p = str(tmp_path / "n_0.hsln")
+    L.write_hsln(p, Q, names)
+    blob = b"".join(n.encode() + b"\0" for n in names)
+    expect = struct.pack("<3i", 37, 4, len(blob)) + blob + Q.tobytes()
+    assert open(p, "rb").read() == expect
+    assert read_hsln(p)[0] == names and np.array_equal(L.read_hsln(p), Q)
+    L.write_hsln(p, Q)
+    assert read_hsln(p)[0] == ["Density", "Velocity_X", "Velocity_Y", "Pressure"]
+    with pytest.raises(ValueError):
+        L.write_hsln(p, Q, names[:3])
 
 
 def read_vtu(path):
