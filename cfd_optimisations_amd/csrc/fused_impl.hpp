@@ -61,8 +61,7 @@ struct FusedTypes<float> {
 // Per-cell inputs of pass A and pass B, read from global memory once.
 template <typename T, int MAXDEG>
 struct CellIn {
-    int nbx[MAXDEG];  // neighbour | twin << 29, or SLOT_EMPTY
-    int lq[MAXDEG];   // the neighbour's local index in this tile's staging (-1: empty slot)
+    int lq[MAXDEG];   // packed local-index entry of each slot (LT_*, halo.hpp), -1: empty slot
     T A[MAXDEG], n0[MAXDEG], n1[MAXDEG], If[MAXDEG], r0[MAXDEG], r1[MAXDEG];
     T iv;
     int side;
@@ -76,7 +75,6 @@ __device__ __forceinline__ void load_cell_in(const PassArgs& a, int g, CellIn<T,
     c.side = plane_at(a.cside, g);
 #pragma unroll
     for (int k = 0; k < MAXDEG; k++) {
-        c.nbx[k] = cnb_at(a, k, g);
         // the mixed path's normal planes hold A n (A unused)
         c.A[k] = sizeof(T) == 8 ? geo<MAXDEG>(gp, P, GEO_A, k, g) : T(0);
         c.n0[k] = geo<MAXDEG>(gp, P, GEO_N0, k, g);
@@ -120,16 +118,11 @@ __device__ __forceinline__ void fused_cell_a(const PassArgs& a, const TileLists&
 #pragma unroll
     for (int k = 0; k < MAXDEG; k++) {
         ln[k] = -1;
-        const int nbx = c.nbx[k];
-        if (nbx == SLOT_EMPTY) continue;
+        const int e = c.lq[k];
+        if (e < 0) continue;  // empty slot
         deg = k + 1;
-        const int nb = nbx & SLOT_NB_MASK;
-        int q = c.lq[k];  // the neighbour's place among the staged cells
-        if (q < 0) {
-            bad = true;
-            q = l;
-        }
-        ln[k] = q;
+        const int q = e & LT_LOCAL_MASK;  // the neighbour's place among the staged cells
+        ln[k] = e;
         double und[NV];
         FusedTypes<double>::get(sU, su_pitch, q, und);
         T un[NV];
@@ -148,7 +141,7 @@ __device__ __forceinline__ void fused_cell_a(const PassArgs& a, const TileLists&
             umin[v] = dmin(umin[v], un[v]);
             umax[v] = dmax(umax[v], un[v]);
         }
-        if (nb >= a.Nint) ghosts |= 1u << k;
+        if (e & LT_GHOST) ghosts |= 1u << k;
     }
 #pragma unroll
     for (int v = 0; v < NV; v++) {
@@ -178,16 +171,17 @@ __device__ __forceinline__ void fused_cell_a(const PassArgs& a, const TileLists&
             if (!DELTA && own && ((ghosts >> k) & 1u)) {
                 // ghost side of a boundary face: U_g + phi_L ((-G_L) . rR), rR = -rL;
                 // the ghost's slot lies in the face-state region [tc + nh1, fs_pitch)
-                if (ln[k] < L.tc + L.nh1 || ln[k] >= fs_pitch) {
+                const int gq = ln[k] & LT_LOCAL_MASK;
+                if (gq < L.tc + L.nh1 || gq >= fs_pitch) {
                     bad = true;
                     continue;
                 }
                 double ug[NV];
                 T ufg[NV];
-                FusedTypes<double>::get(sU, su_pitch, ln[k], ug);
+                FusedTypes<double>::get(sU, su_pitch, gq, ug);
 #pragma unroll
                 for (int v = 0; v < NV; v++) ufg[v] = madd(phi[v], d[v], (T)ug[v]);
-                FT::put(sFS, fs_pitch, ln[k], ufg);
+                FT::put(sFS, fs_pitch, gq, ufg);
             }
         }
     }
@@ -196,9 +190,10 @@ __device__ __forceinline__ void fused_cell_a(const PassArgs& a, const TileLists&
 }
 
 // CFL rate iv sum_f A_f (|u.n_f| + c) of an FP64 state, from a cell's inputs
-// (the cell's bound is 1/rate; see cfl_rate).
+// (the cell's bound is 1/rate; see cfl_rate); bit k of `present`: slot k holds a face.
 template <typename T, int MAXDEG>
-__device__ __forceinline__ double cfl_bound_in(double gamma, const CellIn<T, MAXDEG>& ci, const double u[NV]) {
+__device__ __forceinline__ double cfl_bound_in(double gamma, const CellIn<T, MAXDEG>& ci, const double u[NV],
+                                               unsigned present) {
     const double gmo = gamma - 1;
     const double ir = 1.0 / u[0];
     const double ux = u[1] * ir, uy = u[2] * ir;
@@ -207,7 +202,7 @@ __device__ __forceinline__ double cfl_bound_in(double gamma, const CellIn<T, MAX
     double lam = 0.0;
 #pragma unroll
     for (int k = 0; k < MAXDEG; k++) {
-        if (ci.nbx[k] == SLOT_EMPTY) continue;
+        if (!((present >> k) & 1u)) continue;
         lam += cfl_face(ux, uy, c, ci.A[k], ci.n0[k], ci.n1[k]);
     }
     return cfl_rate((double)ci.iv * lam);
@@ -227,7 +222,11 @@ template <typename T, int MAXDEG>
 __device__ __forceinline__ double cfl_bound(const PassArgs& a, int g, const double u[NV]) {
     CellIn<T, MAXDEG> ci;
     load_cell_in<T, MAXDEG>(a, g, ci);
-    return cfl_bound_in<T, MAXDEG>(a.gamma, ci, u);
+    unsigned present = 0;
+#pragma unroll
+    for (int k = 0; k < MAXDEG; k++)
+        if (cnb_at(a, k, g) != SLOT_EMPTY) present |= 1u << k;
+    return cfl_bound_in<T, MAXDEG>(a.gamma, ci, u, present);
 }
 
 // CFL partials of one tile without a block barrier: each wavefront folds its
@@ -404,15 +403,16 @@ k_fused_step(
             FusedTypes<double>::get(sU, su_pitch, tid, uown);
             FluxBase fb;
             if constexpr (sizeof(T) == 4) fb = flux_base((float)a.gamma, uown);
-            unsigned gmask = 0;
+            unsigned gmask = 0, present = 0;
 #pragma unroll
             for (int k = 0; k < MAXDEG; k++) {
-                const int nbx = ci.nbx[k];
-                if (nbx == SLOT_EMPTY) continue;
-                const int tw = (int)((unsigned)nbx >> SLOT_TWIN_SHIFT);
-                const bool gnb = (nbx & SLOT_NB_MASK) >= a.Nint;
-                int q = lnb[k];
-                if (q < 0 || q >= fs_pitch || tw >= MAXDEG) {
+                const int e = lnb[k];  // packed local-index entry (LT_*)
+                if (e < 0) continue;   // empty slot
+                present |= 1u << k;
+                const int tw = (e >> LT_TWIN_SHIFT) & LT_TWIN_MASK;
+                const bool gnb = (e & LT_GHOST) != 0;
+                int q = e & LT_LOCAL_MASK;
+                if (q >= fs_pitch || tw >= MAXDEG) {
                     bad = true;
                     q = tid;
                 }
@@ -460,8 +460,15 @@ k_fused_step(
 #pragma unroll
             for (int k = 0; k < MAXDEG; k++) {
                 if ((gmask >> k) & 1u) {
+                    // the ghost's cell id: its entry among the tile's ghosts at the
+                    // head of S2 (staged right after the halo ring)
+                    const int gp = (lnb[k] & LT_LOCAL_MASK) - L.tc - L.nh1;
+                    if (gp < 0 || gp >= L.ngt) {
+                        a.fault[inst] = 1;
+                        continue;
+                    }
+                    const int gc = a.s2[s0 + gp];
                     // boundary faces have the interior cell on the L side: n_out = n (FP64)
-                    const int gc = ci.nbx[k] & SLOT_NB_MASK;
                     const BCEntry e = a.bctab[a.ghost_bc[gc - a.Nint]];
                     double ug[NV];
                     ghost_state(e, a.gamma, unew, geo<MAXDEG>(a.cgeo, P, GEO_N0, k, i),
@@ -469,7 +476,7 @@ k_fused_step(
                     store_cell<W, NV>(a.Unext, gc, ug);
                 }
             }
-            if constexpr (CFL) rate = cfl_bound_in<T, MAXDEG>(a.gamma, ci, unew);
+            if constexpr (CFL) rate = cfl_bound_in<T, MAXDEG>(a.gamma, ci, unew, present);
         } else if (bad) {
             a.fault[0] = 1;
         }

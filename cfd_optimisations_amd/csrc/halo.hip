@@ -244,8 +244,15 @@ __global__ void k_local_nb(int Nint, long long P, int nslot, const int* __restri
             const int x = cnb[k * P + c];
             int l = -1;
             if (x != SLOT_EMPTY) {
-                l = local_of(L, x & SLOT_NB_MASK);
-                if (l < 0) miss++;
+                const int nb = x & SLOT_NB_MASK;
+                l = local_of(L, nb);
+                if (l < 0 || l > LT_LOCAL_MASK) {
+                    miss++;
+                    l = -1;
+                } else {
+                    l |= (int)((unsigned)x >> SLOT_TWIN_SHIFT) << LT_TWIN_SHIFT;
+                    if (nb >= Nint) l |= LT_GHOST;
+                }
             }
             if (q < L.tc) lt[k * P + c] = (short)l;
             else lh[k * nh1tot + h0 + (q - L.tc)] = (short)l;
@@ -254,12 +261,12 @@ __global__ void k_local_nb(int Nint, long long P, int nslot, const int* __restri
     if (miss) atomicAdd(unresolved, miss);
 }
 
-// The 16-bit local tables of the final tiling (local indices stay below 2^15
-// for every tiling the single kernel can run: its LDS holds at most a few
-// thousand staged cells).
+// The 16-bit local tables of the final tiling (packed entries, LT_*: a tiling
+// whose tiles stage more than 2^11 cells gets none, and the single-kernel step
+// is unavailable for it; the LDS budgets keep tiles far below that).
 void build_local_tables(int Nint, long long P, int nslot, const int* cnb, HaloTables& out, hipStream_t st) {
     out.unresolved = 0;
-    if (out.ntiles == 0 || out.smax >= (1 << 15)) return;
+    if (out.ntiles == 0 || out.smax > LT_LOCAL_MASK + 1) return;
     const size_t lds = sizeof(int) * ((size_t)out.h1max + out.s2max);
     if (lds > 64 * 1024) return;
     out.lt = dalloc<short>((size_t)nslot * P);

@@ -19,6 +19,16 @@
 
 namespace hcfd {
 
+// Entry of the local-index tables lt / lh for one neighbour slot: the
+// neighbour's local index among the tile's staged cells (bits 0-10), the face's
+// slot in the neighbour's own list (bits 11-13) and whether the neighbour is a
+// ghost (bit 14); -1 for an empty slot.  With these the single-kernel step needs
+// no neighbour planes: a ghost's cell id is its entry of the tile's S2 list.
+constexpr int LT_LOCAL_MASK = (1 << 11) - 1;
+constexpr int LT_TWIN_SHIFT = 11;
+constexpr int LT_TWIN_MASK = 7;
+constexpr int LT_GHOST = 1 << 14;
+
 struct HaloTables {
     int TC = 0, ntiles = 0;
     int* tile_start = nullptr;  // [ntiles + 1] first cell of each tile (tiles hold at most TC cells)
@@ -32,7 +42,7 @@ struct HaloTables {
     int fsmax = 0;          // max over tiles of |T| + |H1| + ghosts of T (staged face states)
     int h1max = 0, s2max = 0;
     long long nh1 = 0, ns2 = 0;
-    // local index (position among the tile's staged cells) of neighbour slot k:
+    // packed entry (LT_* above) of neighbour slot k:
     short* lt = nullptr;    // [nslot][NcPad] of interior cell i in its own tile; -1 for an empty slot
     short* lh = nullptr;    // [nslot][nh1] of H1 entry e (index into h1) in the tile that lists it
     long long unresolved = 0;  // neighbours not found among the staged cells (0 unless the lists are wrong)
