@@ -578,7 +578,11 @@ def run_batch(name: str, steps: int, warmup: int, tune: bool, check: int, compar
                      "timing": "HIP event pairs around eager launches right after the timed region"},
         "faulted_instances": rec["faulted_instances"],
         "err_run": rec.get("err_run"),
+        # plain 16-byte-per-lane copy of 1 GiB each way: the achievable HBM ceiling,
+        # and the known-size kernel the rocprofv3 byte counters are calibrated with
+        "copy_ceiling_gbs": L.copy_bandwidth(1 << 30, 20),
     }
+    line["roofline"]["frac_of_copy_ceiling"] = achieved / line["copy_ceiling_gbs"]
     if check:
         # the smallest instances against the oracle, from the same initial state, same step count
         sys.path.insert(0, os.path.join(ROOT, "tests"))
