@@ -13,6 +13,7 @@
 #include <cstring>
 #include <fstream>
 #include <memory>
+#include <mutex>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -93,6 +94,10 @@ struct hcfd_ctx {
     double* pmin_first = nullptr;
     double* pmin_last = nullptr;
     unsigned long long* dtacc = nullptr;
+    // during hcfd_step_async_noalloc: the same CFL scratch in the static array
+    double* scr_first = nullptr;
+    double* scr_last = nullptr;
+    unsigned long long* scr_acc = nullptr;
     double* dt = nullptr;
     double* tdev = nullptr;
     double* lastdt = nullptr;
@@ -449,9 +454,9 @@ Launch make_launch(hcfd_ctx* c, bool cfl, int cur, int pass) {
     a.FS32 = c->FS32;
     a.inst_off = c->m.inst_int_off;
     a.chunk_inst = c->m.chunk_inst;
-    a.pmin_first = c->pmin_first;
-    a.pmin_last = c->pmin_last;
-    a.dtacc = c->dtacc;
+    a.pmin_first = c->scr_first ? c->scr_first : c->pmin_first;
+    a.pmin_last = c->scr_first ? c->scr_last : c->pmin_last;
+    a.dtacc = c->scr_first ? c->scr_acc : c->dtacc;
     a.dt = c->dt;
     a.bctab = c->bctab;
     a.ghost_bc = c->m.ghost_bc;
@@ -622,6 +627,97 @@ void run_steps(hcfd_ctx* c, int nsteps, double dt, double cfl) {
         for (; done + nbuf(c) <= nsteps; done += nbuf(c)) HIPCHK(hipGraphLaunch(c->gexec, c->st));
     }
     for (; done < nsteps; done++) c->cur = enqueue_step(c, c->cur, cfl);
+}
+
+// ---- allocation-free stepping ---------------------------------------------------
+
+}  // namespace (reopened below)
+
+// CFL scratch of hcfd_step_async_noalloc: the per-tile partial maxima of the
+// CFL rate (tile's first and last instance) and the per-instance atomic maxima
+// that hcfd_step_async keeps in buffers the context allocates.  One copy per
+// GPU, held until the process ends.  Host code never uses this name for the
+// memory: the device address comes from hipGetSymbolAddress (scratch_base).
+constexpr size_t HCFD_NOALLOC_SCRATCH_BYTES = (size_t)16 << 20;
+static __device__ __align__(256) unsigned long long scratch[HCFD_NOALLOC_SCRATCH_BYTES / sizeof(unsigned long long)];
+
+namespace {
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_base[MAX_DEVICES] = {};
+
+// The scratch array's device address on `device` (the current device), looked
+// up once per device.
+void* scratch_base(int device) {
+    if (device < 0 || device >= MAX_DEVICES) fail(HCFD_ERR_ARG, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_base[device]) {
+        void* p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) fail(HCFD_ERR_HIP, "hipGetSymbolAddress returned no address for the CFL scratch");
+        g_scratch_base[device] = p;
+    }
+    return g_scratch_base[device];
+}
+
+__global__ void k_zero_u64(int n, unsigned long long* __restrict__ p) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+__global__ void k_fill_f64(int n, double* __restrict__ p, double x) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = x;
+}
+
+// run_steps with the CFL scratch in the static array, eager launches only (no
+// graph), the fixed dt stored by a kernel: nothing is allocated, on the device
+// or the host, and nothing waits for the device.
+void run_steps_noalloc(hcfd_ctx* c, int nsteps, double dt, double cfl) {
+    require_bcs(c);
+    if (nsteps < 0) fail(HCFD_ERR_ARG, "nsteps < 0");
+    if (!(cfl > 0) && !(dt > 0)) fail(HCFD_ERR_ARG, "need dt > 0 or cfl > 0");
+    // tiles of the schedule's CFL partials: the halo tiles of the single-kernel
+    // step, or pass A's tiles of block / lanes_per_cell cells
+    const long long tiles = c->sched.fused ? (long long)c->halo.ntiles
+                                           : (c->Nint + c->sched.block / c->sched.lanes_per_cell - 1) /
+                                                 (c->sched.block / c->sched.lanes_per_cell);
+    const size_t need = sizeof(double) * (size_t)(2 * tiles + c->ninst);
+    if (need > HCFD_NOALLOC_SCRATCH_BYTES)
+        fail(HCFD_ERR_ARG, "hcfd_step_async_noalloc: this context's schedule needs " + std::to_string(need) +
+                               " bytes of CFL scratch (" + std::to_string(tiles) + " tiles, " +
+                               std::to_string(c->ninst) + " instances); the static array holds " +
+                               std::to_string(HCFD_NOALLOC_SCRATCH_BYTES) + " (use hcfd_step_async)");
+    int dev0 = 0;
+    HIPCHK(hipGetDevice(&dev0));
+    if (dev0 != c->device) HIPCHK(hipSetDevice(c->device));
+    struct Restore {
+        hcfd_ctx* c;
+        int dev0;
+        ~Restore() {
+            c->scr_first = c->scr_last = nullptr;
+            c->scr_acc = nullptr;
+            if (dev0 != c->device) (void)hipSetDevice(dev0);
+        }
+    } restore{c, dev0};
+    double* base = static_cast<double*>(scratch_base(c->device));
+    c->scr_first = base;
+    c->scr_last = base + tiles;
+    c->scr_acc = reinterpret_cast<unsigned long long*>(base + 2 * tiles);
+    // the per-instance maxima start at zero (the array keeps the last call's contents)
+    const int nb = (c->ninst + 255) / 256;
+    k_zero_u64<<<nb, 256, 0, c->st>>>(c->ninst, c->scr_acc);
+    HIPCHK(hipGetLastError());
+    if (!(cfl > 0)) {
+        k_fill_f64<<<nb, 256, 0, c->st>>>(c->ninst, c->dt, dt);
+        HIPCHK(hipGetLastError());
+        c->fixed_dt_loaded = dt;
+    } else {
+        c->fixed_dt_loaded = NAN;  // the CFL reduction overwrites dt
+    }
+    ensure_bound(c, cfl);
+    for (int done = 0; done < nsteps; done++) c->cur = enqueue_step(c, c->cur, cfl);
 }
 
 double device_time(hcfd_ctx* c) {
@@ -1010,6 +1106,13 @@ int hcfd_step_async(hcfd_ctx* c, int nsteps, double dt, double cfl) {
     return guarded([&] {
         if (!c) fail(HCFD_ERR_ARG, "null");
         run_steps(c, nsteps, dt, cfl);
+    });
+}
+
+int hcfd_step_async_noalloc(hcfd_ctx* c, int nsteps, double dt, double cfl) {
+    return guarded([&] {
+        if (!c) fail(HCFD_ERR_ARG, "null");
+        run_steps_noalloc(c, nsteps, dt, cfl);
     });
 }
 

@@ -123,6 +123,7 @@ def load() -> ctypes.CDLL:
         "hcfd_read_hsln": [ctypes.c_char_p, P, P],
         "hcfd_write_vtu": [ctypes.c_char_p, P, P],
         "hcfd_step_async": [P, I, D, D],
+        "hcfd_step_async_noalloc": [P, I, D, D],
         "hcfd_sync": [P, P],
         "hcfd_output_begin": [P, I],
         "hcfd_output_wait": [P, I, P, P],
@@ -414,6 +415,11 @@ class Context:
 
     def step_async(self, nsteps: int, dt: float = 0.0, cfl: float = 0.0) -> None:
         check(load().hcfd_step_async(self.h, nsteps, dt, cfl))
+
+    def step_async_noalloc(self, nsteps: int, dt: float = 0.0, cfl: float = 0.0) -> None:
+        """hcfd_step_async with the CFL scratch in the library's static device
+        array: allocates nothing (eager launches; call sync() to wait)."""
+        check(load().hcfd_step_async_noalloc(self.h, nsteps, dt, cfl))
 
     def sync(self) -> float:
         t = D(0.0)

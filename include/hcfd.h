@@ -208,6 +208,18 @@ int hcfd_step(hcfd_ctx* ctx, int nsteps, double dt, double cfl, double* t_inout)
  * NULL).  Lets the host write output while the device steps on. */
 int hcfd_step_async(hcfd_ctx* ctx, int nsteps, double dt, double cfl);
 int hcfd_sync(hcfd_ctx* ctx, double* t);
+/* No counterpart in the reference.  For callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): hcfd_step_async with nothing allocated, on the device or the
+ * host.  The steps' CFL scratch (per-tile partial maxima and per-instance
+ * atomics, which hcfd_step_async keeps in buffers the context allocated) lives
+ * in one static 16 MiB device array per GPU, held until the process ends; a
+ * context whose schedule needs more (2 x tiles + instances doubles: about 500k
+ * tiles of 256 cells) gets HCFD_ERR_ARG.  Launches are eager (no graph is
+ * instantiated); a fixed dt is stored by a kernel.  Results equal
+ * hcfd_step_async's bit for bit.  Calls that overlap on one GPU share the array
+ * and race: run them on one stream, or one after another. */
+int hcfd_step_async_noalloc(hcfd_ctx* ctx, int nsteps, double dt, double cfl);
 /* Output while stepping (the reference computes primitives and writes them
  * with the solver stopped, Euler::run, src/Base/Euler.cpp:99-103):
  * hcfd_output_begin enqueues Euler::compute_prims (src/Base/Euler.cpp:115-136)

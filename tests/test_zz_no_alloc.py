@@ -1,0 +1,89 @@
+"""The allocation-free step entry point (hcfd_step_async_noalloc, for callers that
+cannot allocate): through the C ABI it equals hcfd_step on the same inputs bit for
+bit, on two calls in a row (the second on a smaller mesh, so the static CFL
+scratch holds the first call's larger contents), in both step paths and in a
+batch (whose per-instance atomic maxima live in the scratch too); a schedule whose
+CFL scratch exceeds the static array is refused."""
+import numpy as np
+import pytest
+
+from cfd_optimisations_amd import cases as C
+from cfd_optimisations_amd import lib as L
+
+pytestmark = pytest.mark.gpu
+
+TWO_PASS = dict(block=256, lanes_per_cell=1, aosoa_width=1, persistent=0, xcd_remap=1, fused=0)
+FUSED = dict(TWO_PASS, fused=1)
+
+
+def pair(case_fn, sched, steps, **kw):
+    """(state, clock, last dt) after `steps` steps through hcfd_step and through
+    hcfd_step_async_noalloc on two contexts of the same case."""
+    out = []
+    for noalloc in (False, True):
+        with case_fn() as ctx:
+            ctx.set_schedule(**sched)
+            if noalloc:
+                ctx.step_async_noalloc(steps, **kw)
+                t = ctx.sync()
+            else:
+                t = ctx.step(steps, **kw)
+            out.append((ctx.get_state(), t, ctx.last_dt(), ctx.fault()))
+    return out
+
+
+@pytest.mark.parametrize("sched", [TWO_PASS, FUSED], ids=["two_pass", "fused"])
+def test_noalloc_step_equals_step(sched):
+    print(f"no-alloc step: Riemann 120x60 then 24x12, schedule {sched}", flush=True)
+    for nx, ny in ((120, 60), (24, 12)):
+        (Ua, ta, da, fa), (Ub, tb, db, fb) = pair(
+            lambda: C.make_context(C.riemann_case(nx, ny, seed=2), L.FP64, L.FLAG_ORDER_SFC), sched, 30, cfl=0.5)
+        assert np.array_equal(Ua, Ub) and ta == tb and da == db and fa == fb == 0, (nx, ny)
+    # fixed dt, then CFL again on the same context: the dt stored by a kernel
+    with C.make_context(C.riemann_case(24, 12, seed=2), L.FP64, L.FLAG_ORDER_SFC) as a, \
+            C.make_context(C.riemann_case(24, 12, seed=2), L.FP64, L.FLAG_ORDER_SFC) as b:
+        for ctx in (a, b):
+            ctx.set_schedule(**sched)
+        a.step(5, dt=1e-4)
+        a.step(5, cfl=0.4)
+        b.step_async_noalloc(5, dt=1e-4)
+        b.step_async_noalloc(5, cfl=0.4)
+        assert b.sync() == a.sync()
+        assert np.array_equal(a.get_state(), b.get_state())
+
+
+def test_noalloc_batch_and_mixed():
+    print("no-alloc step: batch of 12 wedges (FP64), then FP32-mixed 6", flush=True)
+    w = C.WORKLOADS["c4_batch_fp64"]
+    specs = C.wedge_batch_specs(12, 7, w["mach"], w["angle"], (2e3, 2e4))
+    cases = [C.ramp_case(s["nx"], s["ny"], s["mach"], s["angle"], seed=s["seed"]) for s in specs]
+    for prec, n in ((L.FP64, 12), (L.MIXED, 6)):
+        res = []
+        for noalloc in (False, True):
+            with C.make_batch(cases[:n], prec, L.FLAG_ORDER_SFC) as ctx:
+                ctx.set_schedule(**FUSED)
+                if noalloc:
+                    ctx.step_async_noalloc(40, cfl=0.5)
+                    ctx.sync()
+                else:
+                    ctx.step(40, cfl=0.5)
+                res.append(([ctx.inst_state(k) for k in range(n)], ctx.status()))
+        for k in range(n):
+            assert np.array_equal(res[0][0][k], res[1][0][k]), (prec, k)
+        for x, y in zip(res[0][1], res[1][1]):
+            assert np.array_equal(x, y)
+
+
+def test_noalloc_refuses_oversized_scratch():
+    """Pass A tiles of 16 cells (block 64, four lanes per cell) on a 17.6M-cell
+    mesh need 2.2M doubles of CFL partials, more than the 16 MiB array holds."""
+    print("no-alloc step: oversized schedule refused", flush=True)
+    case = C.ramp_case(4200, 2100, 3.0, 20.0, seed=0)
+    with C.make_context(case, L.FP64, L.FLAG_ORDER_SFC) as ctx:
+        ctx.set_schedule(block=64, lanes_per_cell=4, aosoa_width=1, persistent=0, xcd_remap=1, fused=0)
+        with pytest.raises(L.HcfdError, match="CFL scratch"):
+            ctx.step_async_noalloc(1, cfl=0.5)
+        ctx.set_schedule(**FUSED)
+        ctx.step_async_noalloc(2, cfl=0.5)
+        ctx.sync()
+        assert ctx.fault() == 0
