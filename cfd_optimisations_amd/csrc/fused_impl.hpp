@@ -439,7 +439,8 @@ k_fused_step(
 #pragma unroll
                 for (int v = 0; v < NV; v++) {
                     if constexpr (sizeof(T) == 8) R[v] = R[v] - ci.A[k] * F[v];
-                    else K[v].add(-F[v]);  // rusanov_delta returns A (F - E(u).n)
+                    else if (k == 0) K[v].first(-F[v]);  // rusanov_delta returns A (F - E(u).n)
+                    else K[v].add(-F[v]);
                 }
                 if (gnb) gmask |= 1u << k;
             }

@@ -444,9 +444,10 @@ __device__ __forceinline__ void ghost_state(const BCEntry& e, double gma, const 
     }
 }
 
-__device__ __forceinline__ bool nonphysical(double gma, const double u[NV]) {
-    const double p = (gma - 1) * (u[3] - 0.5 * (u[1] * u[1] + u[2] * u[2]) / u[0]);
-    return !(u[0] > 0.0) || !(p > 0.0);
+__device__ __forceinline__ bool nonphysical(double, const double u[NV]) {
+    // rho > 0 and p = (gamma - 1)(E - |m|^2 / (2 rho)) > 0, i.e. 2 rho E > |m|^2
+    // (multiplied through by 2 rho > 0: no division); NaN fails either test
+    return !(u[0] > 0.0) || !(2.0 * u[0] * u[3] > u[1] * u[1] + u[2] * u[2]);
 }
 
 // ---------------------------------------------------------------------------
@@ -548,6 +549,11 @@ __device__ __forceinline__ void pass_b_cell_fp64(const PassArgs& a, int i, doubl
 // ---------------------------------------------------------------------------
 struct Kahan {
     float s = 0.0f, c = 0.0f;
+    // the first term starts the sum exactly (s = x, c = 0)
+    __device__ __forceinline__ void first(float x) {
+        s = x;
+        c = 0.0f;
+    }
     __device__ __forceinline__ void add(float x) {
         const float y = x - c;
         const float t = s + y;
@@ -593,7 +599,10 @@ __device__ __forceinline__ void pass_b_cell_mixed(const PassArgs& a, int i, doub
         rusanov_delta(gma, fb, dO, dN, geo<MAXDEG>(a.cgeo32, P, GEO_N0, k, i), geo<MAXDEG>(a.cgeo32, P, GEO_N1, k, i),
                       F);
 #pragma unroll
-        for (int v = 0; v < NV; v++) R[v].add(-F[v]);
+        for (int v = 0; v < NV; v++) {
+            if (k == 0) R[v].first(-F[v]);  // slot 0 always holds a face
+            else R[v].add(-F[v]);
+        }
         if (nb >= a.Nint) gg[k] = nb;
     }
     double unew[NV];
