@@ -124,6 +124,9 @@ def load() -> ctypes.CDLL:
         "hcfd_write_vtu": [ctypes.c_char_p, P, P],
         "hcfd_step_async": [P, I, D, D],
         "hcfd_step_async_noalloc": [P, I, D, D],
+        "hcfd_rusanov3d": [I, D, P, P, P, P],
+        "hcfd_prims3d": [I, D, P, P],
+        "hcfd_cons3d": [I, D, P, P],
         "hcfd_sync": [P, P],
         "hcfd_output_begin": [P, I],
         "hcfd_output_wait": [P, I, P, P],
@@ -527,3 +530,29 @@ def write_vtu(path: str, mesh, Q: np.ndarray) -> None:
     Q = np.ascontiguousarray(Q, dtype=np.float64)
     check(load().hcfd_write_vtu(path.encode(), ctypes.byref(v), _rows(Q)))
     del keep
+
+
+# ---- the reference's NDIMS == 3 branches (arrays of states; include/hcfd.h) ------
+def rusanov3d(UL: np.ndarray, UR: np.ndarray, nrm: np.ndarray, gamma: float = 1.4) -> np.ndarray:
+    """Flux::Rusanov with the 3-D EulerFlux on the GPU: UL, UR [n][5], nrm [n][3]."""
+    UL, UR, nrm = (np.ascontiguousarray(x, dtype=np.float64) for x in (UL, UR, nrm))
+    n = UL.shape[0]
+    if UL.shape != (n, 5) or UR.shape != (n, 5) or nrm.shape != (n, 3):
+        raise ValueError("UL, UR must be [n][5] and nrm [n][3]")
+    F = np.zeros((n, 5))
+    check(load().hcfd_rusanov3d(n, gamma, _ptr(UL), _ptr(UR), _ptr(nrm), _ptr(F)))
+    return F
+
+
+def prims3d(U: np.ndarray, gamma: float = 1.4) -> np.ndarray:
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    Q = np.zeros_like(U)
+    check(load().hcfd_prims3d(U.shape[0], gamma, _ptr(U), _ptr(Q)))
+    return Q
+
+
+def cons3d(Q: np.ndarray, gamma: float = 1.4) -> np.ndarray:
+    Q = np.ascontiguousarray(Q, dtype=np.float64)
+    U = np.zeros_like(Q)
+    check(load().hcfd_cons3d(Q.shape[0], gamma, _ptr(Q), _ptr(U)))
+    return U

@@ -287,6 +287,20 @@ int hcfd_batch_set_state(hcfd_ctx* ctx, int inst, const double* const U[4]);
 int hcfd_batch_get_status(hcfd_ctx* ctx, double* t, double* last_dt, int* fault);
 int hcfd_batch_instance_size(hcfd_ctx* ctx, int inst, int* Nc, int* Nf);
 
+/* ---- the reference's NDIMS == 3 branches ---------------------------------- */
+/* Five variables (rho, rho u, rho v, rho w, E), AoS arrays of n entries; FP64 with
+ * the reference's expression order (bit-identical to it).  The .hmsh format and
+ * the reference's Domain are two-dimensional, so there is no 3-D mesh to step:
+ * these evaluate the 3-D arithmetic on the GPU for given states.
+ * Flux::Rusanov with the 3-D Flux::EulerFlux (src/Base/Flux.cpp:52-81, 89-104):
+ * F[f] = Rusanov(UL[f], UR[f], nrm[f]) for face-state pairs UL, UR [n][5] and
+ * unit normals nrm [n][3]. */
+int hcfd_rusanov3d(int n, double gamma, const double* UL, const double* UR, const double* nrm, double* F);
+/* Euler::compute_prims (src/Base/Euler.cpp:115-136): U [n][5] -> (rho, u, v, w, p). */
+int hcfd_prims3d(int n, double gamma, const double* U, double* Q);
+/* Euler::initialise's state conversion (src/Base/Euler.cpp:62-68): (rho, u, v, w, p) -> U. */
+int hcfd_cons3d(int n, double gamma, const double* Q, double* U);
+
 /* ---- host mesh services (CPU; used by the C++ host classes and tools) ----- */
 
 typedef struct hcfd_mesh hcfd_mesh;

@@ -41,6 +41,11 @@
  *     temporal order (errors falling as h^2 and h^3) on a smooth flow.
  *   SubInlet/SubOutlet/Wall: closed-form ghost states in the tests (the
  *     golden case only exercises Symmetry, SupInlet and SupOutlet).
+ *   orc_rusanov3d / orc_euler_flux3d / orc_prims3d / orc_cons3d (the NDIMS == 3
+ *     branches, which no mesh of the reference's format can run): with w = 0
+ *     and n_z = 0 they equal the 2-D functions bit for bit (pinned above);
+ *     closed-form Euler flux, Rusanov consistency and antisymmetry, rotation
+ *     invariance, the prim <-> cons round trip.
  */
 #include <math.h>
 #include <stdlib.h>
@@ -558,6 +563,81 @@ double orc_run_ssprk(orc_dom* d, const int* bctype, const double* qbc, double gm
     }
     free(G); free(phi); free(R); free(U0);
     return t;
+}
+
+/* ---------------------------------------------------------------------------
+ * The NDIMS == 3 branches (includes/Global.hpp:20-21 switch the reference to
+ * five variables): state (rho, rho u, rho v, rho w, E), normals of three
+ * components.  The reference's Domain builds normals from two-node faces, so
+ * these are the flux and state conversions alone, on given arrays.
+ * ------------------------------------------------------------------------- */
+static double dot3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+/* Flux::EulerFlux, src/Base/Flux.cpp:89-92, 98-104 (NDIMS == 3). */
+static void euler_flux3(double gma, const double U[5], const double n[3], double F[5]) {
+    double gmo = gma - 1;
+    double ir = 1.0 / U[0];
+    double P = gmo * (U[4] - 0.5 * ir * dot3(&U[1], &U[1]));
+    F[0] = n[0] * U[1] + n[1] * U[2] + n[2] * U[3];
+    F[1] = n[0] * (U[1] * U[1] * ir + P) + n[1] * U[2] * U[1] * ir + n[2] * U[3] * U[1] * ir;
+    F[2] = n[0] * U[1] * U[2] * ir + n[1] * (U[2] * U[2] * ir + P) + n[2] * U[3] * U[2] * ir;
+    F[3] = n[0] * U[1] * U[3] * ir + n[1] * U[2] * U[3] * ir + n[2] * (U[3] * U[3] * ir + P);
+    F[4] = n[0] * (U[4] + P) * U[1] * ir + n[1] * (U[4] + P) * U[2] * ir + n[2] * (U[4] + P) * U[3] * ir;
+}
+
+/* Flux::Rusanov, src/Base/Flux.cpp:52-81, with NDIMS == 3 dot products. */
+static void rusanov3(double gma, const double UL[5], const double UR[5], const double n[3], double F[5]) {
+    double gmo = gma - 1;
+    double irl = 1.0 / UL[0];
+    double irr = 1.0 / UR[0];
+    double nvl = dot3(n, &UL[1]) * irl;
+    double nvr = dot3(n, &UR[1]) * irr;
+    double qql = 0.5 * dot3(&UL[1], &UL[1]) * irl;
+    double qqr = 0.5 * dot3(&UR[1], &UR[1]) * irr;
+    double pl = gmo * (UL[4] - qql);
+    double pr = gmo * (UR[4] - qqr);
+    double al = sqrt(gma * pl * irl);
+    double ar = sqrt(gma * pr * irr);
+    double S1 = fmax(fabs(nvl - al), fabs(nvr - ar));
+    double S2 = fmax(fabs(nvl + al), fabs(nvr + ar));
+    double S = fmax(S1, S2);
+    double FL[5], FR[5];
+    euler_flux3(gma, UL, n, FL);
+    euler_flux3(gma, UR, n, FR);
+    for (int v = 0; v < 5; v++) F[v] = 0.5 * ((FL[v] + FR[v]) - S * (UR[v] - UL[v]));
+}
+
+/* n face-state pairs, AoS [n][5] states and fluxes, [n][3] normals. */
+void orc_rusanov3d(int n, double gma, const double* UL, const double* UR, const double* nrm, double* F) {
+    for (int f = 0; f < n; f++) rusanov3(gma, UL + 5 * (size_t)f, UR + 5 * (size_t)f, nrm + 3 * (size_t)f, F + 5 * (size_t)f);
+}
+
+void orc_euler_flux3d(int n, double gma, const double* U, const double* nrm, double* F) {
+    for (int f = 0; f < n; f++) euler_flux3(gma, U + 5 * (size_t)f, nrm + 3 * (size_t)f, F + 5 * (size_t)f);
+}
+
+/* Euler::compute_prims, src/Base/Euler.cpp:115-136, NDIMS == 3, per state [n][5]. */
+void orc_prims3d(int n, double gma, const double* U, double* Q) {
+    double gmo = gma - 1;
+    for (int i = 0; i < n; i++) {
+        const double* u = U + 5 * (size_t)i;
+        double* q = Q + 5 * (size_t)i;
+        q[0] = u[0];
+        for (int v = 1; v < 4; v++) q[v] = u[v] / u[0];
+        double w[3] = {q[1], q[2], q[3]};
+        q[4] = gmo * (u[4] - 0.5 * q[0] * dot3(w, w));
+    }
+}
+
+/* Euler::initialise's state conversion, src/Base/Euler.cpp:62-68, NDIMS == 3. */
+void orc_cons3d(int n, double gma, const double* Q, double* U) {
+    for (int i = 0; i < n; i++) {
+        const double* q = Q + 5 * (size_t)i;
+        double* u = U + 5 * (size_t)i;
+        u[0] = q[0];
+        for (int v = 1; v < 5; v++) u[v] = q[0] * q[v];
+        u[4] = q[4] / (gma - 1) + 0.5 * q[0] * dot3(&q[1], &q[1]);
+    }
 }
 
 /* Number of steps Euler::run(T, dt, dt_out) takes: t += dt while t < T

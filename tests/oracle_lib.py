@@ -65,6 +65,10 @@ def oracle_lib():
         lib.orc_run_ssprk.argtypes = [P, P, P, D, P, I, D, D, I]
         lib.orc_rusanov.argtypes = [D, P, P, P, P]
         lib.orc_euler_flux.argtypes = [D, P, P, P]
+        lib.orc_rusanov3d.argtypes = [I, D, P, P, P, P]
+        lib.orc_euler_flux3d.argtypes = [I, D, P, P, P]
+        lib.orc_prims3d.argtypes = [I, D, P, P]
+        lib.orc_cons3d.argtypes = [I, D, P, P]
         lib.orc_cons_of_prims.argtypes = [D, P, P]
         lib.orc_count_steps.restype = I
         lib.orc_count_steps.argtypes = [D, D]

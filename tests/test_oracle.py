@@ -493,3 +493,88 @@ def test_initialise_prims_pinned(wedge):
     rel = np.abs(Qb[:, inter] - Q[:, inter]) / np.maximum(np.abs(Q[:, inter]), 1e-300)
     assert rel[0].max() == 0.0  # density is copied
     assert rel[1:3].max() < 4e-16 * 4 and rel[3].max() < 1e-12
+
+
+# ---------------------------------------------------------------------------
+# the NDIMS == 3 branches (src/Base/Flux.cpp:98-104, Euler.cpp:62-68, 115-136)
+# ---------------------------------------------------------------------------
+def random_states3(rng, n):
+    Q = np.stack([0.2 + rng.random(n), rng.uniform(-2, 2, n), rng.uniform(-2, 2, n), rng.uniform(-2, 2, n),
+                  0.1 + rng.random(n)], axis=1)
+    return Q
+
+
+def cons3_np(Q, g=G):
+    U = Q.copy()
+    U[:, 1:4] = Q[:, :1] * Q[:, 1:4]
+    U[:, 4] = Q[:, 4] / (g - 1) + 0.5 * Q[:, 0] * (Q[:, 1:4] ** 2).sum(1)
+    return U
+
+
+def unit3(rng, n):
+    v = rng.standard_normal((n, 3))
+    return v / np.linalg.norm(v, axis=1)[:, None]
+
+
+def call3(name, n, *arrays, out_cols=5):
+    lib = ol.oracle_lib()
+    arrays = [np.ascontiguousarray(a, dtype=np.float64) for a in arrays]
+    out = np.zeros((n, out_cols))
+    getattr(lib, name)(n, G, *[ol._ptr(a) for a in arrays], ol._ptr(out))
+    return out
+
+
+def test_3d_branch_reduces_to_2d_bit_for_bit():
+    """With w = 0 and n_z = 0 every 3-D term the 2-D flux lacks is an exact zero,
+    so the 3-D Rusanov flux, prims and state conversion equal the 2-D oracle's
+    (pinned by the golden files and the reference build) bit for bit."""
+    rng = np.random.default_rng(21)
+    n = 500
+    Q = random_states3(rng, n)
+    Q[:, 3] = 0.0
+    UL, UR = cons3_np(Q), cons3_np(random_states3(rng, n) * [1, 1, 1, 0, 1])
+    nn = unit3(rng, n)
+    nn[:, 2] = 0.0
+    nn[:, :2] /= np.linalg.norm(nn[:, :2], axis=1)[:, None]
+    F3 = call3("orc_rusanov3d", n, UL, UR, nn)
+    idx = [0, 1, 2, 4]
+    for f in range(0, n, 7):
+        F2 = call_flux("orc_rusanov", UL[f, idx], UR[f, idx], nn[f, :2])
+        assert np.array_equal(F3[f, idx], F2) and F3[f, 3] == 0.0
+    Q3 = call3("orc_prims3d", n, UL)
+    U3 = call3("orc_cons3d", n, Q)
+    assert np.array_equal(U3, cons3_np(Q))   # the conversion's closed form, same operation order
+    for f in range(0, n, 50):
+        q2 = Q3[f, idx]
+        u2 = UL[f, idx]
+        assert abs(q2[3] - (G - 1) * (u2[3] - 0.5 * u2[0] * ((u2[1] / u2[0]) ** 2 + (u2[2] / u2[0]) ** 2))) <= 1e-14 * abs(u2[3])
+
+
+def test_3d_rusanov_closed_forms():
+    """Consistency F(U, U, n) = E(U).n (the analytic 3-D Euler flux),
+    antisymmetry F(UL, UR, n) = -F(UR, UL, -n) exactly, invariance under a rotation
+    of the frame, and the prims <-> state round trip."""
+    rng = np.random.default_rng(22)
+    n = 400
+    U = cons3_np(random_states3(rng, n))
+    nn = unit3(rng, n)
+    F = call3("orc_rusanov3d", n, U, U, nn)
+    rho, m, E = U[:, 0], U[:, 1:4], U[:, 4]
+    vel = m / rho[:, None]
+    p = (G - 1) * (E - 0.5 * rho * (vel ** 2).sum(1))
+    un = (vel * nn).sum(1)
+    Fa = np.column_stack([rho * un, m * un[:, None] + p[:, None] * nn, (E + p) * un])
+    np.testing.assert_allclose(F, Fa, rtol=1e-13, atol=1e-13)
+    np.testing.assert_allclose(call3("orc_euler_flux3d", n, U, nn), Fa, rtol=1e-13, atol=1e-13)
+    UR = cons3_np(random_states3(rng, n))
+    assert np.array_equal(call3("orc_rusanov3d", n, U, UR, nn), -call3("orc_rusanov3d", n, UR, U, -nn))
+    # rotation: F(R u, R n) has its momentum rotated by R, its scalars unchanged
+    a, b = 0.7, -1.1
+    Rz = np.array([[np.cos(a), -np.sin(a), 0], [np.sin(a), np.cos(a), 0], [0, 0, 1]])
+    Rx = np.array([[1, 0, 0], [0, np.cos(b), -np.sin(b)], [0, np.sin(b), np.cos(b)]])
+    R = Rz @ Rx
+    rot = lambda X: np.column_stack([X[:, 0], X[:, 1:4] @ R.T, X[:, 4]])
+    Fr = call3("orc_rusanov3d", n, rot(U), rot(UR), nn @ R.T)
+    np.testing.assert_allclose(Fr, rot(call3("orc_rusanov3d", n, U, UR, nn)), rtol=1e-12, atol=1e-12)
+    Q = random_states3(rng, n)
+    np.testing.assert_allclose(call3("orc_prims3d", n, call3("orc_cons3d", n, Q)), Q, rtol=1e-13, atol=1e-14)
