@@ -337,8 +337,8 @@ hcfd_ctx* create_ctx(int n, const hcfd_mesh_view* views, int precision, int flag
         // the share is rounded down to whole HCFD_LDS_GRANULE units: measured, blocks
         // of 53.9 KB (under a third of 160 KB) run two per CU, blocks of 53.5 KB three
         const size_t budget = (size_t)160 * 1024 / blocks / HCFD_LDS_GRANULE * HCFD_LDS_GRANULE;
-        build_halo_tables(c->Nint, c->NcPad, c->deg_bucket, c->m.cnb, 256, precision == HCFD_MIXED, budget, c->halo,
-                          c->st);
+        build_halo_tables(c->Nint, c->NcPad, c->deg_bucket, c->m.cnb, c->m.cside, 256, precision == HCFD_MIXED, budget,
+                          c->halo, c->st);
     }
     c->fused_lds = fused_lds_bytes(precision == HCFD_MIXED, c->deg_bucket, c->halo.smax, c->halo.fsmax);
     {

@@ -64,7 +64,6 @@ struct CellIn {
     int lq[MAXDEG];   // packed local-index entry of each slot (LT_*, halo.hpp), -1: empty slot
     T A[MAXDEG], n0[MAXDEG], n1[MAXDEG], If[MAXDEG], r0[MAXDEG], r1[MAXDEG];
     T iv;
-    int side;
 };
 
 template <typename T, int MAXDEG>
@@ -72,7 +71,6 @@ __device__ __forceinline__ void load_cell_in(const PassArgs& a, int g, CellIn<T,
     const long long P = a.NcPad;
     const T* __restrict__ gp = PassATypes<T>::geo(a);
     c.iv = PassATypes<T>::iv(a, g);
-    c.side = plane_at(a.cside, g);
 #pragma unroll
     for (int k = 0; k < MAXDEG; k++) {
         // the mixed path's normal planes hold A n (A unused)
@@ -130,7 +128,7 @@ __device__ __forceinline__ void fused_cell_a(const PassArgs& a, const TileLists&
         for (int v = 0; v < NV; v++) un[v] = DELTA ? (T)(und[v] - ud[v]) : (T)und[v];
         const T If = c.If[k];
         const T omI = T(1) - If;
-        const bool isR = (c.side >> k) & 1;
+        const bool isR = (e & LT_SIDE_R) != 0;
         const T wo = isR ? omI : If, wn = isR ? If : omI;
 #pragma unroll
         for (int v = 0; v < NV; v++) {

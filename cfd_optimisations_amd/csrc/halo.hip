@@ -216,6 +216,7 @@ void build_lists(int Nint, long long P, int nslot, const int* cnb, const std::ve
 // Local indices of the neighbours of tile t's cells (T, then its H1 entries):
 // one block per tile, the tile's H1 and S2 lists staged in LDS and searched.
 __global__ void k_local_nb(int Nint, long long P, int nslot, const int* __restrict__ cnb,
+                           const int* __restrict__ cside,
                            const int* __restrict__ tile_start, const int* __restrict__ h1_off,
                            const int* __restrict__ h1, const int* __restrict__ s_off, const int* __restrict__ s_ngt,
                            const int* __restrict__ s2, long long nh1tot, short* __restrict__ lt,
@@ -252,6 +253,7 @@ __global__ void k_local_nb(int Nint, long long P, int nslot, const int* __restri
                 } else {
                     l |= (int)((unsigned)x >> SLOT_TWIN_SHIFT) << LT_TWIN_SHIFT;
                     if (nb >= Nint) l |= LT_GHOST;
+                    if ((cside[c] >> k) & 1) l |= LT_SIDE_R;
                 }
             }
             if (q < L.tc) lt[k * P + c] = (short)l;
@@ -262,9 +264,11 @@ __global__ void k_local_nb(int Nint, long long P, int nslot, const int* __restri
 }
 
 // The 16-bit local tables of the final tiling (packed entries, LT_*: a tiling
-// whose tiles stage more than 2^11 cells gets none, and the single-kernel step
-// is unavailable for it; the LDS budgets keep tiles far below that).
-void build_local_tables(int Nint, long long P, int nslot, const int* cnb, HaloTables& out, hipStream_t st) {
+// whose tiles stage more than 2^10 cells gets none, and the single-kernel step
+// is unavailable for it; the LDS budgets keep every tile below that: at most
+// about 900 staged cells in the mixed path, fewer in FP64).
+void build_local_tables(int Nint, long long P, int nslot, const int* cnb, const int* cside, HaloTables& out,
+                        hipStream_t st) {
     out.unresolved = 0;
     if (out.ntiles == 0 || out.smax > LT_LOCAL_MASK + 1) return;
     const size_t lds = sizeof(int) * ((size_t)out.h1max + out.s2max);
@@ -274,7 +278,7 @@ void build_local_tables(int Nint, long long P, int nslot, const int* cnb, HaloTa
     HIPCHK(hipMemsetAsync(out.lt, 0xff, sizeof(short) * (size_t)nslot * P, st));
     unsigned long long* d_miss = dalloc<unsigned long long>(1);
     HIPCHK(hipMemsetAsync(d_miss, 0, sizeof(unsigned long long), st));
-    k_local_nb<<<out.ntiles, 256, lds, st>>>(Nint, P, nslot, cnb, out.tile_start, out.h1_off, out.h1, out.s_off,
+    k_local_nb<<<out.ntiles, 256, lds, st>>>(Nint, P, nslot, cnb, cside, out.tile_start, out.h1_off, out.h1, out.s_off,
                                              out.s_ngt, out.s2, std::max(out.nh1, 1LL), out.lt, out.lh, d_miss);
     HIPCHK(hipGetLastError());
     unsigned long long miss = 0;
@@ -286,7 +290,8 @@ void build_local_tables(int Nint, long long P, int nslot, const int* cnb, HaloTa
 
 }  // namespace
 
-void build_halo_tables(int Nint, int NcPad, int nslot, const int* cnb, int TC, bool mixed, size_t lds_budget,
+void build_halo_tables(int Nint, int NcPad, int nslot, const int* cnb, const int* cside, int TC, bool mixed,
+                       size_t lds_budget,
                        HaloTables& out, hipStream_t st) {
     free_halo_tables(out);
     out.TC = TC;
@@ -330,7 +335,7 @@ void build_halo_tables(int Nint, int NcPad, int nslot, const int* cnb, int TC, b
         out.TC = TC;
     }
     out.split_tiles = out.ntiles - (Nint + TC - 1) / TC;
-    build_local_tables(Nint, NcPad, nslot, cnb, out, st);
+    build_local_tables(Nint, NcPad, nslot, cnb, cside, out, st);
 }
 
 void free_halo_tables(HaloTables& t) {

@@ -20,14 +20,16 @@
 namespace hcfd {
 
 // Entry of the local-index tables lt / lh for one neighbour slot: the
-// neighbour's local index among the tile's staged cells (bits 0-10), the face's
-// slot in the neighbour's own list (bits 11-13) and whether the neighbour is a
-// ghost (bit 14); -1 for an empty slot.  With these the single-kernel step needs
-// no neighbour planes: a ghost's cell id is its entry of the tile's S2 list.
-constexpr int LT_LOCAL_MASK = (1 << 11) - 1;
-constexpr int LT_TWIN_SHIFT = 11;
+// neighbour's local index among the tile's staged cells (bits 0-9), the face's
+// slot in the neighbour's own list (bits 10-12), whether the neighbour is a
+// ghost (bit 13) and whether the cell is the face's R side (bit 14); -1 for an
+// empty slot.  With these the single-kernel step reads neither the neighbour
+// planes nor the side bits: a ghost's cell id is its entry of the tile's S2 list.
+constexpr int LT_LOCAL_MASK = (1 << 10) - 1;
+constexpr int LT_TWIN_SHIFT = 10;
 constexpr int LT_TWIN_MASK = 7;
-constexpr int LT_GHOST = 1 << 14;
+constexpr int LT_GHOST = 1 << 13;
+constexpr int LT_SIDE_R = 1 << 14;
 
 struct HaloTables {
     int TC = 0, ntiles = 0;
@@ -86,7 +88,8 @@ __device__ __forceinline__ int local_of(const TileLists& L, int j) {
 // TC consecutive cells; a tile whose staged cells would need more LDS than
 // lds_budget (the share of one block at the intended occupancy) is halved, and
 // the lists rebuilt, for a few rounds.
-void build_halo_tables(int Nint, int NcPad, int nslot, const int* cnb, int TC, bool mixed, size_t lds_budget,
+void build_halo_tables(int Nint, int NcPad, int nslot, const int* cnb, const int* cside, int TC, bool mixed,
+                       size_t lds_budget,
                        HaloTables& out, hipStream_t st);
 void free_halo_tables(HaloTables& t);
 
