@@ -279,6 +279,27 @@ void Euler::run(double T, double dt, double dt_out) {
     check(hcfd_sync(ctx, nullptr), "hcfd_sync");
 }
 
+void Euler::initialise_prims(const double* const Qin[NVARS]) {
+    check(hcfd_initialise_prims(domain.ctx(), Qin), "hcfd_initialise_prims");
+    check(hcfd_get_state(domain.ctx(), U), "hcfd_get_state");
+}
+
+void Euler::run_cfl(double T, double cfl, double dt_out) {
+    hcfd_ctx* ctx = domain.ctx();
+    double t = 0.0;
+    double t_out = dt_out;
+    compute_prims();
+    writer->write_soln();
+    while (t < T) {
+        check(hcfd_step(ctx, 1, 0.0, cfl, &t), "hcfd_step");
+        if (t >= t_out) {
+            compute_prims();
+            writer->write_soln();
+            t_out += dt_out;
+        }
+    }
+}
+
 // Euler::compute_prims (src/Base/Euler.cpp:115-136), computed on the device.
 void Euler::compute_prims() { check(hcfd_get_prims(domain.ctx(), Q), "hcfd_get_prims"); }
 

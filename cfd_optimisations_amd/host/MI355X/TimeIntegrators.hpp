@@ -25,6 +25,14 @@ class Euler {
     void run(double T, double dt, double dt_out);
     void compute_prims();
 
+    // Per-cell primitive initial state (Euler::initialise's arithmetic cell by
+    // cell, then the BC ghosts): Q[v][i], the mesh's cell order.
+    void initialise_prims(const double* const Q[NVARS]);
+    // run() with the CFL time step (no reference counterpart: the reference steps
+    // a fixed dt): each step's dt is cfl times the CFL bound of its state; files
+    // are written as run() writes them, when the clock passes a multiple of dt_out.
+    void run_cfl(double T, double cfl, double dt_out);
+
     // Output file stem (the reference writes ../sols/<Layout>, src/Base/Euler.cpp:22).
     void set_output(const std::string& fname);
     // Stages per step of run(): 1 forward Euler (the reference's), 2 SSP-RK2,

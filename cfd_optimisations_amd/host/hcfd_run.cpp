@@ -3,6 +3,7 @@
 //
 //   hcfd_run [mesh.hmsh] [output stem] [steps] [--stages] [--outputs K] [--rk S]
 //            [--restart file.hsln] [--mixed]
+//   hcfd_run --case case.ini [--dry-run]
 //
 // Default: ../wedge.hmsh, ../sols/MI355X, 1000 steps of dt = 1e-3, writing
 // <stem>_0.hsln and <stem>_1.hsln as the reference writes sols/Base_{0,1}.hsln.
@@ -13,6 +14,14 @@
 // .hsln file instead of the uniform initial state; --mixed builds the Domain
 // for the FP32-mixed path (MI355X::Domain(mesh, HCFD_MIXED); the per-stage
 // classes of --stages are FP64 only).
+// --case runs the case an INI file describes (CaseFile.hpp): mesh, gamma,
+// precision and cell order, initial state as expressions of the cell centre,
+// one BC section per physical name, integrator, end time, fixed dt or CFL
+// number, output cadence.  --dry-run resolves the case without a GPU: it prints
+// the BC assignment as JSON and writes the initial primitive state of every
+// cell to <stem>_ic.hsln.
+#include <algorithm>
+#include <array>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -28,10 +37,110 @@
 #include "MI355X/GradientSchemes.hpp"
 #include "MI355X/Limiters.hpp"
 #include "MI355X/TimeIntegrators.hpp"
+#include "CaseFile.hpp"
 #include "Mesh.hpp"
+
+namespace {
+
+const char* bc_name(int t) {
+    static const char* n[] = {"symmetry", "wall", "sup-inlet", "sub-inlet", "sup-outlet", "sub-outlet"};
+    return t >= 0 && t < 6 ? n[t] : "?";
+}
+
+int run_case(const std::string& path, bool dry) {
+    const casefile::Case cs = casefile::read_case(path);
+    Mesh mesh(cs.mesh.c_str());
+    // BC ordinal b is colour Ncol - NBC + b; its physical tag t names it pnames[t - 1]
+    // (interface/LEGACY/Run.py:49-71: phystag = physnames.index(name) + 1)
+    const int NBC = mesh.NBCcol, c0 = mesh.Ncol - NBC;
+    int maxtag = 0;
+    for (int c = 0; c < mesh.Ncol; c++) maxtag = std::max(maxtag, mesh.coltag[c]);
+    std::vector<MI355X::BCtype> BCmap(maxtag + 1, MI355X::SUPOUTLET);
+    std::vector<double> qbc(4 * (size_t)std::max(NBC, 1), 0.0);
+    std::string bcjson;
+    std::vector<bool> used_name(mesh.N_pnames, false);
+    for (int b = 0; b < NBC; b++) {
+        const int tag = mesh.coltag[c0 + b];
+        if (tag < 1 || tag > mesh.N_pnames)
+            throw std::runtime_error(cs.mesh + ": boundary colour " + std::to_string(c0 + b) + " has tag " +
+                                     std::to_string(tag) + " with no physical name");
+        const std::string& name = mesh.pnames[tag - 1];
+        auto it = cs.bcs.find(name);
+        if (it == cs.bcs.end()) throw std::runtime_error(path + ": no [BC-" + name + "] section for boundary '" + name + "'");
+        used_name[tag - 1] = true;
+        const casefile::BC& bc = it->second;
+        const bool needs_state = bc.type == HCFD_SUPINLET || bc.type == HCFD_SUBINLET || bc.type == HCFD_SUBOUTLET;
+        if (needs_state && bc.q.empty())
+            throw std::runtime_error(path + ": [BC-" + name + "] " + bc_name(bc.type) + " needs r, u, v, p");
+        BCmap[tag] = static_cast<MI355X::BCtype>(bc.type);
+        for (int v = 0; v < 4; v++) qbc[4 * (size_t)b + v] = bc.q.empty() ? (v == 0 || v == 3 ? 1.0 : 0.0) : bc.q[v](0.0, 0.0);
+        char buf[512];
+        std::snprintf(buf, sizeof buf, "%s{\"ordinal\": %d, \"colour\": %d, \"tag\": %d, \"name\": \"%s\", \"type\": \"%s\", "
+                      "\"q\": [%.17g, %.17g, %.17g, %.17g]}", b ? ", " : "", b, c0 + b, tag, name.c_str(),
+                      bc_name(bc.type), qbc[4 * b], qbc[4 * b + 1], qbc[4 * b + 2], qbc[4 * b + 3]);
+        bcjson += buf;
+    }
+    for (auto& kv : cs.bcs) {
+        bool found = false;
+        for (int k = 0; k < mesh.N_pnames; k++) found = found || (mesh.pnames[k] == kv.first && used_name[k]);
+        if (!found) throw std::runtime_error(path + ": [BC-" + kv.first + "] names no boundary of " + cs.mesh);
+    }
+    // initial primitive state at the cell centres: the node mean, summed as the
+    // reference's Domain does (src/Base/Domain.cpp:61-71)
+    const int Nc = mesh.Nc;
+    std::vector<double> Qb(4 * (size_t)Nc);
+    double* Q[4];
+    for (int v = 0; v < 4; v++) Q[v] = &Qb[(size_t)v * Nc];
+    for (int i = 0; i < Nc; i++) {
+        const int nn = mesh.cn_off[i + 1] - mesh.cn_off[i];
+        double xc = 0.0, yc = 0.0;
+        for (int k = mesh.cn_off[i]; k < mesh.cn_off[i + 1]; k++) {
+            xc += mesh.nodes[2 * mesh.cn_con[k]] / nn;
+            yc += mesh.nodes[2 * mesh.cn_con[k] + 1] / nn;
+        }
+        for (int v = 0; v < 4; v++) Q[v][i] = cs.ic[v](xc, yc);
+    }
+    const double* const Qc[4] = {Q[0], Q[1], Q[2], Q[3]};
+    if (dry) {
+        MI355X::check(hcfd_write_hsln((cs.stem + "_ic.hsln").c_str(), Nc, Qc, nullptr), "hcfd_write_hsln");
+        std::printf("{\"mesh\": \"%s\", \"Nc\": %d, \"gamma\": %.17g, \"precision\": \"%s\", \"order\": \"%s\", "
+                    "\"stages\": %d, \"T\": %.17g, \"dt\": %.17g, \"cfl\": %.17g, \"dt_out\": %.17g, \"stem\": \"%s\", "
+                    "\"bcs\": [%s]}\n",
+                    cs.mesh.c_str(), Nc, cs.gamma, cs.precision == HCFD_MIXED ? "mixed" : "fp64",
+                    cs.hilbert ? "hilbert" : "rcm", cs.stages, cs.T, cs.dt, cs.cfl, cs.dt_out, cs.stem.c_str(),
+                    bcjson.c_str());
+        return 0;
+    }
+    MI355X::Domain domain(mesh, cs.precision, cs.hilbert ? HCFD_FLAG_ORDER_SFC : HCFD_FLAG_NONE);
+    MI355X::GaussGreen grads(domain);
+    MI355X::Limiter limiter(domain);
+    MI355X::Flux flux(4, cs.gamma, domain);
+    MI355X::Euler intg(4, domain, flux, grads, limiter);
+    intg.set_output(cs.stem);
+    intg.gather_BCs(BCmap.data(), reinterpret_cast<const double(*)[NVARS]>(qbc.data()));
+    intg.initialise_prims(Qc);
+    intg.set_stages(cs.stages);
+    auto t1 = std::chrono::steady_clock::now();
+    if (cs.cfl > 0) intg.run_cfl(cs.T, cs.cfl, cs.dt_out);
+    else intg.run(cs.T, cs.dt, cs.dt_out);
+    const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
+    std::printf("MI355X case %s: %d cells, T = %g in %.4f s\n", path.c_str(), Nc, cs.T, s);
+    return 0;
+}
+
+}  // namespace
 
 int main(int argc, char** argv) {
     try {
+        if (argc > 1 && std::strcmp(argv[1], "--case") == 0) {
+            if (argc < 3) throw std::runtime_error("--case needs a file");
+            bool dry = false;
+            for (int k = 3; k < argc; k++) {
+                if (std::strcmp(argv[k], "--dry-run") == 0) dry = true;
+                else throw std::runtime_error(std::string("unknown option ") + argv[k]);
+            }
+            return run_case(argv[2], dry);
+        }
         const char* mesh_path = argc > 1 ? argv[1] : "../wedge.hmsh";
         const std::string out = argc > 2 ? argv[2] : "../sols/MI355X";
         const int niter = argc > 3 ? std::atoi(argv[3]) : 1000;

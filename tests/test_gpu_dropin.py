@@ -131,3 +131,48 @@ def test_dropin_mixed_domain_within_bound(tmp_path, wedge_mesh):
     r = subprocess.run([RUN, os.path.join(GOLDEN, "wedge.hmsh"), str(tmp_path / "x"), "1", "--stages", "--mixed"],
                        capture_output=True, text=True, timeout=60)
     assert r.returncode == 1 and "FP64" in r.stderr
+
+
+def test_case_file_wedge_matches_oracle(tmp_path, wedge_mesh):
+    """src/Test.cpp's case written as a case file (hcfd_run --case): the same
+    files as the oracle's run, bit for bit."""
+    from test_host import WEDGE_CASE
+    p = tmp_path / "wedge.ini"
+    p.write_text(WEDGE_CASE.format(mesh=os.path.join(GOLDEN, "wedge.hmsh"), T=1.0, stem="w"))
+    r = subprocess.run([RUN, "--case", str(p)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    Q0, Q1 = oracle_prims(wedge_mesh, 1000)
+    assert np.array_equal(read_hsln(str(tmp_path / "w_0.hsln"))[1], Q0)
+    assert np.array_equal(read_hsln(str(tmp_path / "w_1.hsln"))[1], Q1)
+
+
+def test_case_file_cfl_ssprk3_matches_oracle(tmp_path):
+    """A case file with expression initial states, SSP-RK3, a CFL number and an
+    output cadence: every file equals the oracle's state at the same step."""
+    from cfd_optimisations_amd import lib as L
+    from cfd_optimisations_amd.formats import read_hmsh
+    from test_host import RIEMANN_CASE, riemann_ic_np
+    nm = L.NativeMesh.riemann(40, 30, 0.2, 6, True)
+    mp = str(tmp_path / "r.hmsh")
+    nm.save(mp)
+    hm = read_hmsh(mp)
+    T, dtout = 0.06, 0.02
+    (tmp_path / "r.ini").write_text(RIEMANN_CASE.format(mesh=mp, prec="fp64", scheme="ssprk3", T=T, dtout=dtout))
+    r = subprocess.run([RUN, "--case", str(tmp_path / "r.ini")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    od = ol.OracleDomain(hm, 1.4, np.full(4, 4), np.tile([1.5, 0.0, 0.0, 1.5], (4, 1)))
+    od.zero_bc_iv()
+    U = od.initialise_prims(riemann_ic_np(od.geometry()["xc"]))
+    files = [od.prims(U)]
+    t, t_out = 0.0, dtout
+    while t < T:
+        U, h = od.run_ssprk(U, 1, 3, cfl=0.4)
+        t += h
+        if t >= t_out:
+            files.append(od.prims(U))
+            t_out += dtout
+    assert len(files) >= 4 and np.isfinite(files[-1]).all()
+    for k, Qo in enumerate(files):
+        _, Qg = read_hsln(str(tmp_path / f"r_{k}.hsln"))
+        assert np.array_equal(Qg, Qo), k
+    assert not os.path.exists(tmp_path / f"r_{len(files)}.hsln")

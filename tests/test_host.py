@@ -399,3 +399,149 @@ def test_bench_refuses_several_gpus():
                        text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr and r.stdout == ""
+
+
+# ---------------------------------------------------------------------------
+# case files (hcfd_run --case, the form of interface/LEGACY/Run.py's INI)
+# ---------------------------------------------------------------------------
+RUNNER = os.path.join(ROOT, "cfd_optimisations_amd", "hcfd_run")
+
+WEDGE_CASE = """; the reference's Test.cpp case (src/Test.cpp:51-74)
+[mesh]
+file = {mesh}
+[consts]
+gamma = 1.4
+[ICs]
+r = 1
+u = 1
+v = 0
+p = 1/(1.4*2^2)
+[BC-upper]
+type = symmetry
+[BC-lower]
+type = symmetry
+[BC-inlet]
+type = sup-inlet
+r = 1
+u = 1
+v = 0
+p = 1/(1.4*4)
+[BC-outlet]
+type = sup-outlet
+[time-integrator]
+scheme = euler
+T = {T}
+dt = 1e-3
+[output]
+stem = {stem}
+"""
+
+# four quadrant states written as expressions of the cell centre (Lax-Liu
+# configuration 3 states in the placement of cases.riemann_case); a saved
+# generated mesh names its BC colours bc0..bc3 (bottom, right, top, left)
+RIEMANN_CASE = """[mesh]
+file = {mesh}
+[solver]
+precision = {prec}
+order = rcm
+[ICs]
+r = 1.5*(x>=0.5)*(y>=0.5) + 0.138*(x<0.5)*(y>=0.5) + 0.5323*(x<0.5)*(y<0.5) + 0.5323*(x>=0.5)*(y<0.5)
+u = 1.206*(x<0.5)*(y>=0.5) + 1.206*(x>=0.5)*(y<0.5)
+v = 1.206*(x<0.5)*(y>=0.5) + 1.206*(x<0.5)*(y<0.5)
+p = 1.5*(x>=0.5)*(y>=0.5) + 0.029*(x<0.5)*(y>=0.5) + 0.3*(x<0.5)*(y<0.5) + 0.3*(x>=0.5)*(y<0.5)
+[BC-bc0]
+type = sup-outlet
+[BC-bc1]
+type = sup-outlet
+[BC-bc2]
+type = sup-outlet
+[BC-bc3]
+type = sup-outlet
+[time-integrator]
+scheme = {scheme}
+T = {T}
+cfl = 0.4
+dt-out = {dtout}
+"""
+
+
+def riemann_ic_np(xc):
+    """The expressions of RIEMANN_CASE evaluated the same way, term by term."""
+    x, y = xc[:, 0], xc[:, 1]
+    a, b, c, d = ((x >= 0.5) & (y >= 0.5)), ((x < 0.5) & (y >= 0.5)), ((x < 0.5) & (y < 0.5)), ((x >= 0.5) & (y < 0.5))
+    f = lambda m: m.astype(np.float64)
+    r = 1.5 * f(x >= 0.5) * f(y >= 0.5) + 0.138 * f(x < 0.5) * f(y >= 0.5) + 0.5323 * f(x < 0.5) * f(y < 0.5) \
+        + 0.5323 * f(x >= 0.5) * f(y < 0.5)
+    u = 1.206 * f(x < 0.5) * f(y >= 0.5) + 1.206 * f(x >= 0.5) * f(y < 0.5)
+    v = 1.206 * f(x < 0.5) * f(y >= 0.5) + 1.206 * f(x < 0.5) * f(y < 0.5)
+    p = 1.5 * f(x >= 0.5) * f(y >= 0.5) + 0.029 * f(x < 0.5) * f(y >= 0.5) + 0.3 * f(x < 0.5) * f(y < 0.5) \
+        + 0.3 * f(x >= 0.5) * f(y < 0.5)
+    assert (a | b | c | d).all()
+    return np.stack([r, u, v, p])
+
+
+def run_case(path, *extra):
+    import subprocess
+    return subprocess.run([RUNNER, "--case", str(path)] + list(extra), capture_output=True, text=True, timeout=300)
+
+
+def test_case_file_dry_run(tmp_path, wedge_path):
+    """A case file resolves without a GPU: BCs by physical name onto BC ordinals as
+    Euler::gather_BCs maps them (src/Base/Euler.cpp:42-60), the initial state at
+    every cell centre (the node mean, summed as src/Base/Domain.cpp:61-71 does)."""
+    import json
+    p = tmp_path / "wedge.ini"
+    p.write_text(WEDGE_CASE.format(mesh=wedge_path, T=1.0, stem="w"))
+    r = run_case(p, "--dry-run")
+    assert r.returncode == 0, r.stderr
+    d = json.loads(r.stdout)
+    assert (d["Nc"], d["stages"], d["dt"], d["cfl"], d["T"], d["dt_out"]) == (3740, 1, 1e-3, 0.0, 1.0, 1.0)
+    assert [(b["name"], b["type"]) for b in d["bcs"]] == [("upper", "symmetry"), ("lower", "symmetry"),
+                                                         ("inlet", "sup-inlet"), ("outlet", "sup-outlet")]
+    assert d["bcs"][2]["q"] == [1.0, 1.0, 0.0, 1.0 / (1.4 * 2 * 2)]
+    _, Q = read_hsln(str(tmp_path / "w_ic.hsln"))
+    assert np.array_equal(Q, np.tile(np.array([[1.0], [1.0], [0.0], [1.0 / (1.4 * 4)]]), (1, 3740)))
+    # expressions of x, y on a generated mesh, bit for bit against numpy at the oracle's centres
+    nm = L.NativeMesh.riemann(24, 16, 0.2, 5, True)
+    mp = str(tmp_path / "r.hmsh")
+    nm.save(mp)
+    hm = read_hmsh(mp)
+    (tmp_path / "r.ini").write_text(RIEMANN_CASE.format(mesh=mp, prec="fp64", scheme="ssprk3", T=0.05, dtout=0.025))
+    r = run_case(tmp_path / "r.ini", "--dry-run")
+    assert r.returncode == 0, r.stderr
+    d = json.loads(r.stdout)
+    assert d["order"] == "rcm" and d["stages"] == 3 and d["cfl"] == 0.4 and d["stem"].endswith("/r")
+    _, Q = read_hsln(str(tmp_path / "r_ic.hsln"))
+    xc = ol.OracleDomain(hm).geometry()["xc"]
+    assert np.array_equal(Q, riemann_ic_np(xc))
+
+
+@pytest.mark.parametrize("change,message", [
+    (("type = sup-outlet\n[time-integrator]", "type = outflow\n[time-integrator]"), "unknown BC type"),
+    (("[BC-upper]", "[BC-top]"), "no [BC-upper] section"),
+    (("[BC-outlet]\ntype = sup-outlet", "[BC-outlet]\ntype = sup-outlet\n[BC-side]\ntype = wall"), "names no boundary"),
+    (("r = 1\nu = 1\nv = 0\np = 1/(1.4*2^2)", "r = 1\nu = 1\nv = 0\np = 1/(1.4*"), "expression"),
+    (("dt = 1e-3", "dt = 1e-3\ncfl = 0.5"), "exactly one of dt"),
+    (("p = 1/(1.4*2^2)", "p = sqr(2)"), "unknown name"),
+    (("[BC-inlet]\ntype = sup-inlet\nr = 1\nu = 1\nv = 0\np = 1/(1.4*4)", "[BC-inlet]\ntype = sup-inlet"), "needs r, u, v, p"),
+])
+def test_case_file_errors(tmp_path, wedge_path, change, message):
+    text = WEDGE_CASE.format(mesh=wedge_path, T=1.0, stem="w")
+    assert change[0] in text
+    p = tmp_path / "bad.ini"
+    p.write_text(text.replace(change[0], change[1], 1))
+    r = run_case(p, "--dry-run")
+    assert r.returncode == 1 and message in r.stderr, r.stderr
+
+
+def test_case_file_expression_precedence(tmp_path, wedge_path):
+    """-2^2 = -4, 2^3^2 = 2^9, comparisons give 1 or 0 and bind loosest, functions."""
+    text = WEDGE_CASE.format(mesh=wedge_path, T=1.0, stem="e").replace(
+        "r = 1\nu = 1\nv = 0\np = 1/(1.4*2^2)",
+        "r = 2^3^2 - 500\nu = -2^2 + 3*(1 < 2) - (2 <= 1)\nv = max(abs(-3), sqrt(16)) + min(1, 2) * pow(2, 3)\n"
+        "p = (x > -1) * exp(0) + 10 - 4 / 2 * 3")
+    (tmp_path / "e.ini").write_text(text)
+    r = run_case(tmp_path / "e.ini", "--dry-run")
+    assert r.returncode == 0, r.stderr
+    _, Q = read_hsln(str(tmp_path / "e_ic.hsln"))
+    assert np.array_equal(Q[:, 0], [12.0, -1.0, 12.0, 5.0])
