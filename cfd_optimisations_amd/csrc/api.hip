@@ -138,9 +138,14 @@ struct hcfd_ctx {
 #ifndef HCFD_LDS_GRANULE
 #define HCFD_LDS_GRANULE 2048
 #endif
-// Blocks per CU the FP32-mixed single-kernel step's LDS budget is sized for.
+// Blocks per CU the single-kernel step's LDS budget is sized for: FP32-mixed on
+// triangles (five, the waves per SIMD its register budget allows; four on other
+// cells), and FP64 on cells of at most four faces (octagon meshes: two).
 #ifndef HCFD_FUSED_MIXED_BLOCKS
-#define HCFD_FUSED_MIXED_BLOCKS 4
+#define HCFD_FUSED_MIXED_BLOCKS 5
+#endif
+#ifndef HCFD_FUSED_FP64_BLOCKS
+#define HCFD_FUSED_FP64_BLOCKS 3
 #endif
 
 namespace {
@@ -331,9 +336,11 @@ hcfd_ctx* create_ctx(int n, const hcfd_mesh_view* views, int precision, int flag
     c->sched.xcd_remap = 1;
     // halo tiles of the single-kernel step; used when their LDS footprint fits
     // LDS share of one block at the occupancy the registers allow: FP64 three
-    // blocks of four wavefronts per CU (two for octagon meshes), FP32-mixed four
+    // blocks of four wavefronts per CU (two for octagon meshes), FP32-mixed five
+    // on triangles (four otherwise)
     {
-        const int blocks = precision == HCFD_MIXED ? HCFD_FUSED_MIXED_BLOCKS : (c->deg_bucket <= 4 ? 3 : 2);
+        const int blocks = precision == HCFD_MIXED ? (c->deg_bucket <= 3 ? HCFD_FUSED_MIXED_BLOCKS : 4)
+                                                   : (c->deg_bucket <= 4 ? HCFD_FUSED_FP64_BLOCKS : 2);
         // the share is rounded down to whole HCFD_LDS_GRANULE units: measured, blocks
         // of 53.9 KB (under a third of 160 KB) run two per CU, blocks of 53.5 KB three
         const size_t budget = (size_t)160 * 1024 / blocks / HCFD_LDS_GRANULE * HCFD_LDS_GRANULE;

@@ -272,14 +272,16 @@ __device__ __forceinline__ void cfl_tile_lds(const PassArgs& a, int tile, int in
     }
 }
 
-// Minimum waves per SIMD for the single-kernel step (bounds its VGPRs; the
-// LDS of a tile allows about three FP64 blocks of four waves per CU, four in
-// FP32-mixed; octagon meshes two).
+// Minimum waves per SIMD for the single-kernel step on triangles (bounds its
+// VGPRs; the LDS of a tile allows three FP64 blocks of four waves per CU and
+// five FP32-mixed ones; cells of more faces two).  FP32-mixed at five waves
+// spills 16-40 B per lane and is still 9% faster on the 1,024-wedge batch than
+// at four (measured 7.49 against 8.25 ms), neutral on the 16M-triangle ramp.
 #ifndef HCFD_WAVES_FUSED
 #define HCFD_WAVES_FUSED 3
 #endif
 #ifndef HCFD_WAVES_FUSED_MIXED
-#define HCFD_WAVES_FUSED_MIXED 3
+#define HCFD_WAVES_FUSED_MIXED 5
 #endif
 
 template <typename T, int W, int MAXDEG, bool CFL, bool MULTI>
