@@ -178,21 +178,51 @@ __device__ __forceinline__ T limiter_face(T d, T umin_minus_u, T umax_minus_u) {
     return take ? dmin(T(1), q) : T(1);
 }
 
+// The limiter of one variable over all faces of a cell, the minimum of
+// limiter_face(d_f), from dpos = max(0, max_f d_f) and dneg = min(0, min_f d_f).
+// For d > eps the candidate (Umax-U)/d has a numerator >= 0 and falls as d
+// grows; for d < -eps, (Umin-U)/d has a numerator <= 0 and falls as |d| grows.
+// A correctly rounded quotient keeps that order, so the minimum over the faces
+// is the candidate of the largest d and the one of the most negative d: two
+// divides per variable instead of one per face, and the same value (faces with
+// |d| <= eps, NaN d and NaN quotients drop out as they do face by face).
+template <typename T>
+__device__ __forceinline__ T limiter_phi(T dpos, T dneg, T umin_minus_u, T umax_minus_u) {
+    const T eps = T(1e-9);
+    T phi = T(1);
+    if (dpos > eps) phi = dmin(phi, quot(umax_minus_u, dpos));
+    if (dneg < -eps) phi = dmin(phi, quot(umin_minus_u, dneg));
+    return phi;
+}
+
 // Element i of a per-cell plane: the plane's base is uniform across the wave
 // and i enters as a 32-bit byte offset, so each load is one scalar base plus
 // the lane's offset (global_load's saddr form) with no 64-bit vector add per
 // plane.  Needs i * sizeof(T) < 2^32: contexts hold at most 2^29 cells.
+// The empty asm pins the base in scalar registers and hides how it was formed:
+// without it the compiler folds the plane offsets of neighbouring planes into
+// 64-bit vector address arithmetic (one v_mad_i64_i32 per plane load).  The
+// base must be wave-uniform (the asm reads it from scalar registers).
 template <typename T>
 __device__ __forceinline__ T plane_at(const T* __restrict__ base, int i) {
-    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (unsigned)i * (unsigned)sizeof(T));
+    using GT = __attribute__((address_space(1))) const T;
+    using GB = __attribute__((address_space(1))) const char;
+    GT* b = (GT*)base;
+    asm("" : "+s"(b));
+    return *(GT*)((GB*)b + (unsigned)i * (unsigned)sizeof(T));
 }
 
 // Cell-state access with an AoSoA tile of width W: component c of cell i of an
 // array with NC components lives at (i/W)*NC*W + c*W + i%W.  W == 1 is AoS,
-// W >= the cell count behaves as SoA.
+// W >= the cell count behaves as SoA.  The cell's first element is formed in
+// 32-bit unsigned arithmetic (NC * cells < 2^32: contexts hold at most 2^29
+// cells and NC <= 4), so W a power of two costs a shift and a mask, and c * W
+// stays a constant the loads take as an immediate offset.
 template <int W, int NC>
 __device__ __forceinline__ long long cidx(int i, int c) {
-    return (long long)(i / W) * (NC * W) + c * W + (i % W);
+    const unsigned u = (unsigned)i;
+    const unsigned x = (u / W) * (NC * W) + u % W;
+    return (long long)x + c * W;
 }
 
 template <int W, int NC, typename T>

@@ -86,8 +86,8 @@ __device__ __forceinline__ void load_cell_in(const PassArgs& a, int g, CellIn<T,
 // Pass A of one staged cell (local index l): gradient, limiter and the face
 // state of each of its faces into LDS (a tile cell also writes the ghost side
 // of its boundary faces).  lnb_out receives the local neighbour indices.
-// Empty slots carry zero geometry (the packing writes zeros), so skipping them
-// by nbx leaves every sum exactly as in the two-pass kernels.
+// Empty slots carry zero geometry (the packing writes zeros), so every sum and
+// bound is exactly that of the two-pass kernels, which skip them.
 template <typename T, int MAXDEG>
 __device__ __forceinline__ void fused_cell_a(const PassArgs& a, const TileLists& L, const CellIn<T, MAXDEG>& c, int l,
                                              bool own, const double2* __restrict__ sU, int su_pitch,
@@ -110,17 +110,19 @@ __device__ __forceinline__ void fused_cell_a(const PassArgs& a, const TileLists&
         umin[v] = u[v];
         umax[v] = u[v];
     }
+    // Cells of at most four faces run every slot without a branch: an empty
+    // slot takes the cell itself as its neighbour, and with its zero geometry
+    // it adds exact zeros to the gradient, leaves min/max as they are and gives
+    // d = 0 in the limiter.  Larger cells skip their empty slots.
+    constexpr bool DENSE = MAXDEG <= 4;
     int ln[MAXDEG];
     unsigned ghosts = 0;
-    int deg = 0;
 #pragma unroll
     for (int k = 0; k < MAXDEG; k++) {
-        ln[k] = -1;
         const int e = c.lq[k];
-        if (e < 0) continue;  // empty slot
-        deg = k + 1;
-        const int q = e & LT_LOCAL_MASK;  // the neighbour's place among the staged cells
-        ln[k] = e;
+        ln[k] = e;  // -1: empty slot
+        if (!DENSE && e < 0) continue;
+        const int q = e < 0 ? l : (e & LT_LOCAL_MASK);  // the neighbour's place among the staged cells
         double und[NV];
         FusedTypes<double>::get(sU, su_pitch, q, und);
         T un[NV];
@@ -139,7 +141,7 @@ __device__ __forceinline__ void fused_cell_a(const PassArgs& a, const TileLists&
             umin[v] = dmin(umin[v], un[v]);
             umax[v] = dmax(umax[v], un[v]);
         }
-        if (e & LT_GHOST) ghosts |= 1u << k;
+        if (e >= 0 && (e & LT_GHOST)) ghosts |= 1u << k;
     }
 #pragma unroll
     for (int v = 0; v < NV; v++) {
@@ -149,15 +151,20 @@ __device__ __forceinline__ void fused_cell_a(const PassArgs& a, const TileLists&
     T phi[NV];
 #pragma unroll
     for (int v = 0; v < NV; v++) {
-        phi[v] = T(1);
-        const T dmn = umin[v] - u[v], dmx = umax[v] - u[v];
+        // an empty slot's r is zero, so its d = 0 changes neither bound
+        T dpos = T(0), dneg = T(0);
 #pragma unroll
-        for (int k = 0; k < MAXDEG; k++)
-            if (k < deg) phi[v] = dmin(phi[v], limiter_face(dot_r(g2[v][0], g2[v][1], c.r0[k], c.r1[k]), dmn, dmx));
+        for (int k = 0; k < MAXDEG; k++) {
+            if (!DENSE && ln[k] < 0) continue;
+            const T d = dot_r(g2[v][0], g2[v][1], c.r0[k], c.r1[k]);
+            dpos = dmax(dpos, d);
+            dneg = dmin(dneg, d);
+        }
+        phi[v] = limiter_phi(dpos, dneg, umin[v] - u[v], umax[v] - u[v]);
     }
 #pragma unroll
     for (int k = 0; k < MAXDEG; k++) {
-        if (k < deg) {
+        if (DENSE || ln[k] >= 0) {  // an empty slot's face state is never read
             T d[NV], uf[NV];
 #pragma unroll
             for (int v = 0; v < NV; v++) {
@@ -398,7 +405,6 @@ k_fused_step(
             lane_inst = inst;
             const double dt = a.dt[inst];
             double R[NV] = {0.0, 0.0, 0.0, 0.0};
-            Kahan K[NV];
             // U^n of the cell back from LDS (not held in registers through pass A)
             FusedTypes<double>::get(sU, su_pitch, tid, uown);
             FluxBase fb;
@@ -439,21 +445,15 @@ k_fused_step(
 #pragma unroll
                 for (int v = 0; v < NV; v++) {
                     if constexpr (sizeof(T) == 8) R[v] = R[v] - ci.A[k] * F[v];
-                    else if (k == 0) K[v].first(-F[v]);  // rusanov_delta returns A (F - E(u).n)
-                    else K[v].add(-F[v]);
+                    else R[v] = R[v] - (double)F[v];  // rusanov_delta returns A (F - E(u).n)
                 }
                 if (gnb) gmask |= 1u << k;
             }
             double unew[NV];
-            if constexpr (sizeof(T) == 8) {
 #pragma unroll
-                for (int v = 0; v < NV; v++) {
-                    R[v] = R[v] * (double)ci.iv;
-                    unew[v] = uown[v] + dt * R[v];
-                }
-            } else {
-#pragma unroll
-                for (int v = 0; v < NV; v++) unew[v] = uown[v] + dt * (K[v].value() * (double)ci.iv);
+            for (int v = 0; v < NV; v++) {
+                R[v] = R[v] * (double)ci.iv;
+                unew[v] = uown[v] + dt * R[v];
             }
             if (a.U0) rk_combine<W>(a, i, unew);
             store_cell<W, NV>(a.Unext, i, unew);

@@ -257,15 +257,16 @@ __device__ __forceinline__ double pass_a_cell(const PassArgs& a, int i, const do
     T phi[NV];
 #pragma unroll
     for (int v = 0; v < NV; v++) {
-        phi[v] = T(1);
-        const T dmn = umin[v] - u[v], dmx = umax[v] - u[v];
+        T dpos = T(0), dneg = T(0);
 #pragma unroll
         for (int k = 0; k < MAXDEG; k++) {
             if (k < deg) {
                 const T d = dot_r(g[v][0], g[v][1], r0[k], r1[k]);
-                phi[v] = dmin(phi[v], limiter_face(d, dmn, dmx));
+                dpos = dmax(dpos, d);
+                dneg = dmin(dneg, d);
             }
         }
+        phi[v] = limiter_phi(dpos, dneg, umin[v] - u[v], umax[v] - u[v]);
     }
     // MUSCL face states on this cell's side, U + phi (G . r)
     T* FS = PT::fs(a);
@@ -355,11 +356,16 @@ __device__ __forceinline__ double pass_a_cell_quad(const PassArgs& a, int i, int
     }
     g0 *= iv;
     g1 *= iv;
-    T phi = T(1);
-    const T dmn = umin - u, dmx = umax - u;
+    T dpos = T(0), dneg = T(0);
 #pragma unroll
-    for (int k = 0; k < MAXDEG; k++)
-        if (k < deg) phi = dmin(phi, limiter_face(dot_r(g0, g1, r0[k], r1[k]), dmn, dmx));
+    for (int k = 0; k < MAXDEG; k++) {
+        if (k < deg) {
+            const T d = dot_r(g0, g1, r0[k], r1[k]);
+            dpos = dmax(dpos, d);
+            dneg = dmin(dneg, d);
+        }
+    }
+    const T phi = limiter_phi(dpos, dneg, umin - u, umax - u);
     T* FS = PT::fs(a);
 #pragma unroll
     for (int k = 0; k < MAXDEG; k++) {
@@ -545,24 +551,9 @@ __device__ __forceinline__ void pass_b_cell_fp64(const PassArgs& a, int i, doubl
 // ---------------------------------------------------------------------------
 // Pass B, FP32-mixed: flux in FP32 in difference form (rusanov_delta) from the
 // FP32 face-state increments and the FP64 neighbour differences, residual
-// summed with Kahan compensation, update and ghost states in FP64.
+// summed in FP64 (each FP32 term exactly, one rounding per addition), update
+// and ghost states in FP64.
 // ---------------------------------------------------------------------------
-struct Kahan {
-    float s = 0.0f, c = 0.0f;
-    // the first term starts the sum exactly (s = x, c = 0)
-    __device__ __forceinline__ void first(float x) {
-        s = x;
-        c = 0.0f;
-    }
-    __device__ __forceinline__ void add(float x) {
-        const float y = x - c;
-        const float t = s + y;
-        c = (t - s) - y;
-        s = t;
-    }
-    __device__ __forceinline__ double value() const { return (double)s - (double)c; }
-};
-
 template <int W, int MAXDEG>
 __device__ __forceinline__ void pass_b_cell_mixed(const PassArgs& a, int i, double dt, int inst,
                                                   const float4* __restrict__ sF, int tb, int tc) {
@@ -572,7 +563,7 @@ __device__ __forceinline__ void pass_b_cell_mixed(const PassArgs& a, int i, doub
     const float iv = a.iv32[i];
     const float gma = (float)a.gamma;
     const FluxBase fb = flux_base(gma, u);
-    Kahan R[NV];
+    double R[NV] = {0.0, 0.0, 0.0, 0.0};
     int gg[MAXDEG];
 #pragma unroll
     for (int k = 0; k < MAXDEG; k++) {
@@ -599,15 +590,12 @@ __device__ __forceinline__ void pass_b_cell_mixed(const PassArgs& a, int i, doub
         rusanov_delta(gma, fb, dO, dN, geo<MAXDEG>(a.cgeo32, P, GEO_N0, k, i), geo<MAXDEG>(a.cgeo32, P, GEO_N1, k, i),
                       F);
 #pragma unroll
-        for (int v = 0; v < NV; v++) {
-            if (k == 0) R[v].first(-F[v]);  // slot 0 always holds a face
-            else R[v].add(-F[v]);
-        }
+        for (int v = 0; v < NV; v++) R[v] = R[v] - (double)F[v];
         if (nb >= a.Nint) gg[k] = nb;
     }
     double unew[NV];
 #pragma unroll
-    for (int v = 0; v < NV; v++) unew[v] = u[v] + dt * (R[v].value() * (double)iv);
+    for (int v = 0; v < NV; v++) unew[v] = u[v] + dt * (R[v] * (double)iv);
     if (a.U0) rk_combine<W>(a, i, unew);
     store_cell<W, NV>(a.Unext, i, unew);
     if (nonphysical(a.gamma, unew)) a.fault[inst] = 1;
