@@ -72,9 +72,11 @@ __device__ __forceinline__ void rusanov(T gma, const T UL[NV], const T UR[NV], T
     const T pr = gmo * (UR[3] - qqr);
     const T al = vsqrt(gma * pl * irl);
     const T ar = vsqrt(gma * pr * irr);
-    const T S1 = dmax(fabs(nvl - al), fabs(nvr - ar));
-    const T S2 = dmax(fabs(nvl + al), fabs(nvr + ar));
-    const T S = dmax(S1, S2);
+    // S = max(|nvl - al|, |nvr - ar|, |nvl + al|, |nvr + ar|) as the reference
+    // writes it; with a >= 0, max(|q - a|, |q + a|) = |q| + a, and rounding is
+    // symmetric and monotone, so the rounded values agree as well (a NaN sound
+    // speed drops out of both forms, leaving the other side's bound)
+    const T S = dmax(fabs(nvl) + al, fabs(nvr) + ar);
     T FL[NV], FR[NV];
     euler_flux(gma, UL, irl, n0, n1, FL);
     euler_flux(gma, UR, irr, n0, n1, FR);
@@ -124,9 +126,10 @@ __device__ __forceinline__ FluxBase flux_base(float gma, const double u[NV]) {
 // dE(d) of one face state u + d (see above) and its normal velocity qs and
 // sound speed cs.  The normal N = (N0, N1) is the face's A n (length times unit
 // normal: the mixed path's geometry planes hold it), so every output is A times
-// its unit-normal value; a2 = |N|^2 = A^2 and q = u.N of the owning cell.
+// its unit-normal value; ga2 = gamma |N|^2 = gamma A^2 and q = u.N of the
+// owning cell.
 __device__ __forceinline__ void euler_flux_delta(float gma, const FluxBase& b, float q, const float d[NV], float n0,
-                                                 float n1, float a2, float dF[NV], float& qs, float& cs) {
+                                                 float n1, float ga2, float dF[NV], float& qs, float& cs) {
     const float irs = recip(b.rho + d[0]);                  // 1 / rho_side
     const float dmn = madd(d[1], n0, d[2] * n1);            // (m_side - m).N
     const float dq = madd(-q, d[0], dmn) * irs;             // q_side - q
@@ -134,25 +137,26 @@ __device__ __forceinline__ void euler_flux_delta(float gma, const FluxBase& b, f
     // |m_side|^2/rho_side - |m|^2/rho = ((2m + dm).dm - (m.u) drho) / rho_side
     const float dmu = madd(madd(2.0f, b.mx, d[1]), d[1], madd(madd(2.0f, b.my, d[2]), d[2], -(b.mu * d[0]))) * irs;
     const float dp = (gma - 1.0f) * madd(-0.5f, dmu, d[3]);  // p_side - p
-    cs = vsqrt(gma * (b.p + dp) * irs * a2);                // A c_side
+    cs = vsqrt((b.p + dp) * irs * ga2);                     // A c_side
     dF[0] = dmn;
     dF[1] = madd(d[1], qs, madd(b.mx, dq, dp * n0));
     dF[2] = madd(d[2], qs, madd(b.my, dq, dp * n1));
     dF[3] = madd(d[3] + dp, qs, b.ep * dq);
 }
 
-// A (F - E(u).n) of one face (see above) with N = A n_out, the outward frame of
-// the owning cell.
+// 2 A (F - E(u).n) of one face (see above) with N = A n_out, the outward frame
+// of the owning cell: the factor 1/2 of the Rusanov flux is left to the caller,
+// which applies it once to the sum over the cell's faces (exactly: a power of two).
 __device__ __forceinline__ void rusanov_delta(float gma, const FluxBase& b, const float dO[NV], const float dN[NV],
                                               float n0, float n1, float F[NV]) {
     const float q = madd(b.ux, n0, b.uy * n1);
-    const float a2 = madd(n0, n0, n1 * n1);
+    const float ga2 = gma * madd(n0, n0, n1 * n1);
     float FO[NV], FN[NV], qo, co, qn, cn;
-    euler_flux_delta(gma, b, q, dO, n0, n1, a2, FO, qo, co);
-    euler_flux_delta(gma, b, q, dN, n0, n1, a2, FN, qn, cn);
-    const float S = dmax(dmax(fabsf(qo - co), fabsf(qn - cn)), dmax(fabsf(qo + co), fabsf(qn + cn)));
+    euler_flux_delta(gma, b, q, dO, n0, n1, ga2, FO, qo, co);
+    euler_flux_delta(gma, b, q, dN, n0, n1, ga2, FN, qn, cn);
+    const float S = dmax(fabsf(qo) + co, fabsf(qn) + cn);  // as in rusanov
 #pragma unroll
-    for (int v = 0; v < NV; v++) F[v] = 0.5f * madd(-S, dN[v] - dO[v], FO[v] + FN[v]);
+    for (int v = 0; v < NV; v++) F[v] = madd(-S, dN[v] - dO[v], FO[v] + FN[v]);
 }
 
 // One face's term of the CFL rate, A (|u.n| + c): FP64 from FP64 geometry (the

@@ -24,6 +24,7 @@
 #pragma once
 #include <type_traits>
 
+#include "dispatch.hpp"
 #include "halo.hpp"
 #include "passes_impl.hpp"
 
@@ -291,7 +292,10 @@ __device__ __forceinline__ void cfl_tile_lds(const PassArgs& a, int tile, int in
 #define HCFD_WAVES_FUSED_MIXED 5
 #endif
 
-template <typename T, int W, int MAXDEG, bool CFL, bool MULTI>
+// LOOP: a block walks several tiles (a persistent grid); otherwise each block
+// owns at most one tile, and the kernel holds no loop, so its register budget
+// serves that one tile only (the loop's variant needs more registers).
+template <typename T, int W, int MAXDEG, bool CFL, bool MULTI, bool LOOP>
 __global__ __launch_bounds__(FUSED_TILE) __attribute__((amdgpu_waves_per_eu(
     MAXDEG > 3 ? 2 : (sizeof(T) == 4 ? HCFD_WAVES_FUSED_MIXED : HCFD_WAVES_FUSED)))) void
 k_fused_step(
@@ -445,13 +449,14 @@ k_fused_step(
 #pragma unroll
                 for (int v = 0; v < NV; v++) {
                     if constexpr (sizeof(T) == 8) R[v] = R[v] - ci.A[k] * F[v];
-                    else R[v] = R[v] - (double)F[v];  // rusanov_delta returns A (F - E(u).n)
+                    else R[v] = R[v] - (double)F[v];  // rusanov_delta returns 2 A (F - E(u).n)
                 }
                 if (gnb) gmask |= 1u << k;
             }
             double unew[NV];
 #pragma unroll
             for (int v = 0; v < NV; v++) {
+                if constexpr (sizeof(T) == 4) R[v] *= 0.5;  // the Rusanov 1/2 of the mixed terms
                 R[v] = R[v] * (double)ci.iv;
                 unew[v] = uown[v] + dt * R[v];
             }
@@ -486,11 +491,43 @@ k_fused_step(
     // One tile per block (the default grid) runs outside any loop: inside one,
     // the compiler hoists the tile cell's LDS addresses out of it and, in FP64,
     // spills them (measured 36-76 B per lane and 20% slower).
-    if (tfirst + step >= tend) {
-        if (tfirst < tend) run_tile(tfirst);
-    } else {
+    if constexpr (LOOP) {
         for (int tile = tfirst; tile < tend; tile += step) run_tile(tile);
+    } else {
+        if (tfirst + step < tend) {  // a grid of fewer blocks than tiles: launch_fused_step_t picks LOOP
+            if (tid == 0) a.fault[0] = 1;
+            return;
+        }
+        if (tfirst < tend) run_tile(tfirst);
     }
+}
+
+// Launch of the single-kernel step, one tile per block or (LOOP) a persistent
+// grid; explicitly instantiated in fused_{fp64,mixed}{,_loop}.hip so the four
+// variant sets compile in parallel.
+template <typename T, bool LOOP>
+hipError_t launch_fused_step_t(const PassVariant& v, const PassArgs& a, int grid, size_t lds, hipStream_t st) {
+    return with_width(v.width, [&](auto W) {
+        return with_degree(v.maxdeg, [&](auto MD) {
+            return with_bool(v.cfl, [&](auto CF) {
+                return with_bool(a.ninst > 1, [&](auto MU) {
+                    auto k = k_fused_step<T, decltype(W)::value, decltype(MD)::value, decltype(CF)::value,
+                                          decltype(MU)::value, LOOP>;
+                    hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                       (int)lds);
+                    if (e != hipSuccess) return e;
+                    hipLaunchKernelGGL(k, dim3(grid), dim3(FUSED_TILE), lds, st, a);
+                    return hipGetLastError();
+                });
+            });
+        });
+    });
+}
+
+// True when `grid` blocks leave some block more than one tile (tile_walk).
+inline bool fused_grid_loops(const PassArgs& a, int grid) {
+    const long long one_each = a.xcd_remap ? (long long)NXCD * a.tiles_per_xcd : (long long)a.ntiles;
+    return (long long)grid < one_each;
 }
 
 // CFL bound of the current state U (tile partials only), before the first

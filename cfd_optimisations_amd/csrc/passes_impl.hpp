@@ -586,7 +586,7 @@ __device__ __forceinline__ void pass_b_cell_mixed(const PassArgs& a, int i, doub
         }
 #pragma unroll
         for (int v = 0; v < NV; v++) dN[v] += (float)(un[v] - u[v]);
-        float F[NV];  // A (F - E(u).n): the mixed geometry planes hold A n
+        float F[NV];  // 2 A (F - E(u).n): the mixed geometry planes hold A n
         rusanov_delta(gma, fb, dO, dN, geo<MAXDEG>(a.cgeo32, P, GEO_N0, k, i), geo<MAXDEG>(a.cgeo32, P, GEO_N1, k, i),
                       F);
 #pragma unroll
@@ -595,7 +595,7 @@ __device__ __forceinline__ void pass_b_cell_mixed(const PassArgs& a, int i, doub
     }
     double unew[NV];
 #pragma unroll
-    for (int v = 0; v < NV; v++) unew[v] = u[v] + dt * (R[v] * (double)iv);
+    for (int v = 0; v < NV; v++) unew[v] = u[v] + dt * ((0.5 * R[v]) * (double)iv);
     if (a.U0) rk_combine<W>(a, i, unew);
     store_cell<W, NV>(a.Unext, i, unew);
     if (nonphysical(a.gamma, unew)) a.fault[inst] = 1;
