@@ -612,6 +612,15 @@ def run_batch(name: str, steps: int, warmup: int, tune: bool, check: int, compar
     ctx.close()
     if compare_fp64 and prec == L.MIXED:
         ctx64, rec64, S64, _ = one(L.FP64, timed=True, keep=slice(0, 4))
+        sched64 = rec64["schedule"]
+        line["fp64_batch"] = {"value": rec64["value"], "ms_per_step": rec64["ms_per_step"],
+                              "kernel_ms": rec64["kernel_ms"], "schedule": sched64,
+                              "faulted_instances": rec64["faulted_instances"], "err_run": rec64.get("err_run")}
+        line["speedup_vs_fp64_batch"] = rec["value"] / rec64["value"]
+        if S64 is None:  # --err-steps 0: timings only
+            ctx64.close()
+            compare_fp64 = False
+    if compare_fp64 and prec == L.MIXED:
         names = ("rho", "rho_u", "rho_v", "rho_E")
         rel = np.full((4, ctx64.ninst), np.nan)
         mabs = np.zeros(4)
@@ -624,15 +633,10 @@ def run_batch(name: str, steps: int, warmup: int, tune: bool, check: int, compar
             rel[:, k] = np.linalg.norm(d, axis=1) / np.linalg.norm(Ua_k, axis=1)
             fin = np.isfinite(d).all(axis=1)
             mabs = np.where(fin, np.maximum(mabs, np.nanmax(np.abs(d), axis=1)), mabs)
-        sched64 = rec64["schedule"]
         ctx64.close()
         del Um
         e = rel[0]
         fin = np.isfinite(e)
-        line["fp64_batch"] = {"value": rec64["value"], "ms_per_step": rec64["ms_per_step"],
-                              "kernel_ms": rec64["kernel_ms"], "schedule": sched64,
-                              "faulted_instances": rec64["faulted_instances"], "err_run": rec64.get("err_run")}
-        line["speedup_vs_fp64_batch"] = rec["value"] / rec64["value"]
         line["error_vs_fp64"] = {
             "what": f"per instance, relative L2 of each conserved variable of the FP32-mixed batch against the "
                     f"FP64 batch after the same {err_steps} CFL steps from the same initial state; max_abs over all "

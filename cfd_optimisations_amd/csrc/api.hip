@@ -358,6 +358,9 @@ hcfd_ctx* create_ctx(int n, const hcfd_mesh_view* views, int precision, int flag
         // halo than they update; Hilbert tiles keep the halo near a quarter
         c->fused_default = c->fused_ok && c->halo.split_tiles * 20 <= c->halo.ntiles;
     }
+    if (c->fused_ok)
+        build_ring_inputs(c->deg_bucket, (long long)P, precision == HCFD_MIXED ? nullptr : c->m.cgeo, c->m.iv,
+                          c->cgeo32, c->iv32, c->halo, c->st);
     c->sched.fused = c->fused_default ? 1 : 0;
     HIPCHK(hipStreamSynchronize(c->st));
     return c.release();
@@ -476,6 +479,10 @@ Launch make_launch(hcfd_ctx* c, bool cfl, int cur, int pass) {
     a.lt = c->halo.lt;
     a.lh = c->halo.lh;
     a.nh1tot = std::max(c->halo.nh1, 1LL);
+    a.hgeo = c->halo.hgeo;
+    a.hiv = c->halo.hiv;
+    a.hgeo32 = c->halo.hgeo32;
+    a.hiv32 = c->halo.hiv32;
     a.smax = c->halo.smax;
     a.fsmax = c->halo.fsmax;
     a.tacc = c->tdev;
@@ -907,6 +914,9 @@ int hcfd_get_info(hcfd_ctx* c, hcfd_info* o) {
         b += (c->FS ? 32LL : c->FS32 ? 16LL : 0) * (long long)P * c->deg_bucket;
         b += (c->cgeo32 ? 4LL * GEO_PLANES * c->deg_bucket * P + P * 4 : 0);
         b += 8LL * c->Nc * 8 + 16LL * c->Nf + 16LL * c->Nc;  // staging, xf, xc
+        const long long nh1 = std::max(c->halo.nh1, 1LL);       // ring input copy
+        if (c->halo.hgeo) b += 8LL * (GEO_PLANES * c->deg_bucket + 1) * nh1;
+        if (c->halo.hgeo32) b += 4LL * (GEO_PLANES * c->deg_bucket + 1) * nh1;
         o->device_bytes = b;
         o->sched = c->sched;
         o->fused_available = c->fused_ok ? 1 : 0;

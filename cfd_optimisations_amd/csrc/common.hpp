@@ -34,9 +34,10 @@ struct alignas(64) FaceRec64 {
 };
 static_assert(sizeof(FaceRec64) == 64, "FaceRec64 must be 64 bytes");
 
-// Per-slot geometry planes of the fused passes (cell-major, one plane per
-// quantity and slot): plane p of slot k of cell i lives at
-// (p * nslot + k) * NcPad + i.  n is the face normal oriented OUT of cell i
+// Per-slot geometry planes of the fused passes (one plane per quantity and
+// slot, stored in blocks of 64 cells: plane p of slot k of cell i lives at
+// ((i / 64) * GEO_PLANES * nslot + p * nslot + k) * 64 + i % 64, geo_index in
+// device_math.hpp).  n is the face normal oriented OUT of cell i
 // (the L->R normal, negated when i is the face's R cell); r is the face centre
 // minus the centre of cell i (rL or rR); If is the face's weight of U_L.
 enum { GEO_A = 0, GEO_N0, GEO_N1, GEO_IF, GEO_R0, GEO_R1, GEO_PLANES };

@@ -207,6 +207,16 @@ __device__ __forceinline__ T limiter_phi(T dpos, T dneg, T umin_minus_u, T umax_
 // without it the compiler folds the plane offsets of neighbouring planes into
 // 64-bit vector address arithmetic (one v_mad_i64_i32 per plane load).  The
 // base must be wave-uniform (the asm reads it from scalar registers).
+// Element (plane p, slot k) of cell i in the per-slot geometry arrays (plane
+// order GEO_*): blocks of 64 consecutive cells, each holding its GEO_PLANES x
+// nslot planes of 64 values back to back.  A wavefront's load of one plane is
+// still one contiguous run, and one per-lane address reaches every plane of a
+// cell through immediate offsets instead of one scalar base per plane.  The
+// arrays hold a whole number of blocks (cell counts padded to 64).
+__host__ __device__ __forceinline__ long long geo_index(int nslot, int p, int k, long long i) {
+    return ((i >> 6) * (GEO_PLANES * nslot) + (p * nslot + k)) * 64 + (i & 63);
+}
+
 template <typename T>
 __device__ __forceinline__ T plane_at(const T* __restrict__ base, int i) {
     using GT = __attribute__((address_space(1))) const T;

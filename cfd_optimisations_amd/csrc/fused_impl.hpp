@@ -67,11 +67,12 @@ struct CellIn {
     T iv;
 };
 
+// Entry g of planes of pitch P: a cell's own planes (g = cell id, P = NcPad)
+// or the copy of the tile's ring inputs (g = H1 entry, P = nh1tot).
 template <typename T, int MAXDEG>
-__device__ __forceinline__ void load_cell_in(const PassArgs& a, int g, CellIn<T, MAXDEG>& c) {
-    const long long P = a.NcPad;
-    const T* __restrict__ gp = PassATypes<T>::geo(a);
-    c.iv = PassATypes<T>::iv(a, g);
+__device__ __forceinline__ void load_cell_in(const T* __restrict__ gp, const T* __restrict__ ivp, long long P, int g,
+                                             CellIn<T, MAXDEG>& c) {
+    c.iv = plane_at(ivp, g);
 #pragma unroll
     for (int k = 0; k < MAXDEG; k++) {
         // the mixed path's normal planes hold A n (A unused)
@@ -82,6 +83,15 @@ __device__ __forceinline__ void load_cell_in(const PassArgs& a, int g, CellIn<T,
         c.r0[k] = geo<MAXDEG>(gp, P, GEO_R0, k, g);
         c.r1[k] = geo<MAXDEG>(gp, P, GEO_R1, k, g);
     }
+}
+template <typename T, int MAXDEG>
+__device__ __forceinline__ void load_cell_in(const PassArgs& a, int g, CellIn<T, MAXDEG>& c) {
+    load_cell_in<T, MAXDEG>(PassATypes<T>::geo(a), PassATypes<T>::ivp(a), a.NcPad, g, c);
+}
+// Inputs of H1 entry e (the ring cell a.h1[e]) from the ring copy.
+template <typename T, int MAXDEG>
+__device__ __forceinline__ void load_ring_in(const PassArgs& a, int e, CellIn<T, MAXDEG>& c) {
+    load_cell_in<T, MAXDEG>(PassATypes<T>::ring_geo(a), PassATypes<T>::ring_iv(a), a.nh1tot, e, c);
 }
 
 // Pass A of one staged cell (local index l): gradient, limiter and the face
@@ -355,7 +365,7 @@ k_fused_step(
         const int qh = FUSED_TILE - 1 - rtid;
         CellIn<T, MAXDEG> ch;
         if (early_ring && qh < L.nh1) {
-            load_cell_in<T, MAXDEG>(a, a.h1[h0 + qh], ch);
+            load_ring_in<T, MAXDEG>(a, h0 + qh, ch);
             load_local(a.lh, a.nh1tot, h0 + qh, ch);
         }
         // staged halo cell q = tid (ring H1, then S2): its id and state are in
@@ -391,7 +401,7 @@ k_fused_step(
         if (hasT) fused_cell_a<T, MAXDEG>(a, L, ci, tid, true, sU, su_pitch, sFS, fs_pitch, lnb, bad);
         for (int q = qh; q < L.nh1; q += FUSED_TILE) {
             if (!early_ring || q != qh) {
-                load_cell_in<T, MAXDEG>(a, a.h1[h0 + q], ch);
+                load_ring_in<T, MAXDEG>(a, h0 + q, ch);
                 load_local(a.lh, a.nh1tot, h0 + q, ch);
             }
             int ln[MAXDEG];

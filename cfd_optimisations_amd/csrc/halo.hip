@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "device_math.hpp"
 #include "halo.hpp"
 
 namespace hcfd {
@@ -338,9 +339,45 @@ void build_halo_tables(int Nint, int NcPad, int nslot, const int* cnb, const int
     build_local_tables(Nint, NcPad, nslot, cnb, cside, out, st);
 }
 
+namespace {
+
+// Entry e of the ring copy: the geometry of the H1 cell h1[e] at entry e of
+// the same 64-cell block layout (geo_index), its 1/V at e.
+template <typename T>
+__global__ void k_ring_inputs(int nslot, long long nh1, const int* __restrict__ h1, const T* __restrict__ geo,
+                              const T* __restrict__ iv, T* __restrict__ hgeo, T* __restrict__ hiv) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nh1) return;
+    const long long g = h1[e];
+    for (int p = 0; p < GEO_PLANES; p++)
+        for (int k = 0; k < nslot; k++) hgeo[geo_index(nslot, p, k, e)] = geo[geo_index(nslot, p, k, g)];
+    hiv[e] = iv[g];
+}
+
+}  // namespace
+
+void build_ring_inputs(int nslot, long long P, const double* cgeo, const double* iv, const float* cgeo32,
+                       const float* iv32, HaloTables& t, hipStream_t st) {
+    (void)P;
+    const long long nh1 = (std::max(t.nh1, 1LL) + 63) / 64 * 64;  // whole 64-entry blocks
+    const size_t ngeo = (size_t)GEO_PLANES * nslot * nh1;
+    if (cgeo) {
+        t.hgeo = dalloc<double>(ngeo);
+        t.hiv = dalloc<double>((size_t)nh1);
+        if (t.nh1 > 0)
+            k_ring_inputs<double><<<nblk(t.nh1), 256, 0, st>>>(nslot, t.nh1, t.h1, cgeo, iv, t.hgeo, t.hiv);
+    } else {
+        t.hgeo32 = dalloc<float>(ngeo);
+        t.hiv32 = dalloc<float>((size_t)nh1);
+        if (t.nh1 > 0)
+            k_ring_inputs<float><<<nblk(t.nh1), 256, 0, st>>>(nslot, t.nh1, t.h1, cgeo32, iv32, t.hgeo32, t.hiv32);
+    }
+    HIPCHK(hipGetLastError());
+}
+
 void free_halo_tables(HaloTables& t) {
     for (void* p : {(void*)t.tile_start, (void*)t.h1_off, (void*)t.h1, (void*)t.s_off, (void*)t.s_ngt, (void*)t.s2,
-                    (void*)t.lt, (void*)t.lh})
+                    (void*)t.lt, (void*)t.lh, (void*)t.hgeo, (void*)t.hiv, (void*)t.hgeo32, (void*)t.hiv32})
         if (p) (void)hipFree(p);
     t = HaloTables{};
 }

@@ -48,6 +48,15 @@ struct HaloTables {
     short* lt = nullptr;    // [nslot][NcPad] of interior cell i in its own tile; -1 for an empty slot
     short* lh = nullptr;    // [nslot][nh1] of H1 entry e (index into h1) in the tile that lists it
     long long unresolved = 0;  // neighbours not found among the staged cells (0 unless the lists are wrong)
+    // The inputs of every H1 entry, copied next to the lists in the context's
+    // precision: per-slot geometry in the cell arrays' 64-entry block layout
+    // (geo_index, entries padded to a multiple of 64) and 1/V.  A
+    // block reads its ring's inputs as contiguous runs instead of gathering
+    // them at scattered cell ids (build_ring_inputs).
+    double* hgeo = nullptr;
+    double* hiv = nullptr;
+    float* hgeo32 = nullptr;
+    float* hiv32 = nullptr;
 };
 
 // The lists of one tile as the kernels see them, and the local index of cell j
@@ -92,6 +101,12 @@ void build_halo_tables(int Nint, int NcPad, int nslot, const int* cnb, const int
                        size_t lds_budget,
                        HaloTables& out, hipStream_t st);
 void free_halo_tables(HaloTables& t);
+
+// Copies the inputs of every H1 entry into hgeo/hiv (FP64 planes cgeo, iv) or
+// hgeo32/hiv32 (FP32-mixed planes cgeo32, iv32: pass null cgeo) from the
+// per-slot planes of pitch P.
+void build_ring_inputs(int nslot, long long P, const double* cgeo, const double* iv, const float* cgeo32,
+                       const float* iv32, HaloTables& t, hipStream_t st);
 
 // Bytes of dynamic LDS the single-kernel step needs (block of 256 lanes):
 // staged states (FP64, 32 B, in both precisions), face states (FP64: 32 B,

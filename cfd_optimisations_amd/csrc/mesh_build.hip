@@ -482,7 +482,7 @@ __global__ void k_pack_cellgeo(int NcPad, int nslot, const Slot* __restrict__ sl
             if (isR) side |= 1 << k;
         }
         cnb[k * P + i] = nbx;
-        for (int p = 0; p < GEO_PLANES; p++) cgeo[((long long)p * nslot + k) * P + i] = g[p];
+        for (int p = 0; p < GEO_PLANES; p++) cgeo[geo_index(nslot, p, k, i)] = g[p];
     }
     cside[i] = side;
 }
@@ -500,9 +500,11 @@ __global__ void k_permute_d(int n, const int* __restrict__ old_of_new, const dou
 __global__ void k_geo_mixed(int nslot, long long P, const double* __restrict__ g, float* __restrict__ g32) {
     const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (long long)nslot * P) return;
-    const long long a = (long long)GEO_A * nslot * P + e;
+    const int k = (int)(e / P);
+    const long long i = e - k * P;
+    const long long a = geo_index(nslot, GEO_A, k, i);
     for (int p = 0; p < GEO_PLANES; p++) {
-        const long long q = (long long)p * nslot * P + e;
+        const long long q = geo_index(nslot, p, k, i);
         g32[q] = (p == GEO_N0 || p == GEO_N1) ? (float)(g[a] * g[q]) : (float)g[q];
     }
 }

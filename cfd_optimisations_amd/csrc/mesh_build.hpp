@@ -38,7 +38,7 @@ struct DeviceMesh {
     FaceRec64* faces = nullptr;     // [Nf], new numbering
     int* cnb = nullptr;             // [nslot][NcPad] neighbour | twin slot << 29, SLOT_EMPTY
     int* cside = nullptr;           // [NcPad] bit k: the cell is the R side of its slot-k face
-    double* cgeo = nullptr;         // [GEO_PLANES][nslot][NcPad] per-slot geometry
+    double* cgeo = nullptr;         // per-slot geometry, GEO_PLANES x nslot planes in 64-cell blocks (geo_index)
     double* iv = nullptr;           // [NcPad], new numbering, ghosts 0
     int* ghost_bc = nullptr;        // [max(Nghost,1)] BC table entry per ghost
     int* inst_int_off = nullptr;    // [ninst+1] interior ranges, new numbering

@@ -19,7 +19,7 @@ struct PassArgs {
     double gamma = 1.4;
     const int* cnb = nullptr;         // [MAXDEG][NcPad] neighbour | twin slot << 29, or SLOT_EMPTY
     const int* cside = nullptr;       // [NcPad] side bits (bit k: R side of slot k's face)
-    const double* cgeo = nullptr;     // [GEO_PLANES][MAXDEG][NcPad] per-slot geometry, FP64
+    const double* cgeo = nullptr;     // per-slot geometry, FP64, GEO_PLANES x MAXDEG planes in 64-cell blocks (geo_index)
     const float* cgeo32 = nullptr;    // same, FP32 (mixed path; the normal planes hold A n)
     const double* iv = nullptr;
     const float* iv32 = nullptr;
@@ -46,6 +46,10 @@ struct PassArgs {
     const short* lt = nullptr;          // [nslot][NcPad] local index of each neighbour in the cell's tile
     const short* lh = nullptr;          // [nslot][nh1tot] the same for each H1 entry
     long long nh1tot = 1;
+    const double* hgeo = nullptr;       // per-slot geometry of each H1 entry (geo_index layout), FP64
+    const double* hiv = nullptr;        // [nh1tot]
+    const float* hgeo32 = nullptr;      // the same, FP32-mixed
+    const float* hiv32 = nullptr;
     int smax = 0, fsmax = 0;
     double* tacc = nullptr;             // [ninst] simulated time
     double* lastdt = nullptr;           // [ninst] dt of the latest step

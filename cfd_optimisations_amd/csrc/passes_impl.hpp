@@ -133,10 +133,11 @@ __device__ __forceinline__ long long fs_index(long long P, int k, int i, int v) 
     if constexpr (sizeof(T) == 8) return (((2 * k + (v >> 1)) * P + i) << 1) + (v & 1);
     else return ((k * P + i) << 2) + v;
 }
-// Geometry plane p of slot k of cell i (MAXDEG slot planes per quantity).
+// Geometry plane p of slot k of cell i (MAXDEG slot planes per quantity, in
+// 64-cell blocks: geo_index; the pitch argument is not needed by that layout).
 template <int MAXDEG, typename T>
-__device__ __forceinline__ T geo(const T* __restrict__ g, long long P, int p, int k, int i) {
-    return plane_at(g + (long long)(p * MAXDEG + k) * P, i);
+__device__ __forceinline__ T geo(const T* __restrict__ g, long long, int p, int k, int i) {
+    return g[geo_index(MAXDEG, p, k, (unsigned)i)];
 }
 
 // Slot k of cell i in the neighbour table.
@@ -155,12 +156,18 @@ struct PassATypes<double> {
     __device__ static const double* geo(const PassArgs& a) { return a.cgeo; }
     __device__ static double* fs(const PassArgs& a) { return a.FS; }
     __device__ static double iv(const PassArgs& a, int i) { return plane_at(a.iv, i); }
+    __device__ static const double* ivp(const PassArgs& a) { return a.iv; }
+    __device__ static const double* ring_geo(const PassArgs& a) { return a.hgeo; }
+    __device__ static const double* ring_iv(const PassArgs& a) { return a.hiv; }
 };
 template <>
 struct PassATypes<float> {
     __device__ static const float* geo(const PassArgs& a) { return a.cgeo32; }
     __device__ static float* fs(const PassArgs& a) { return a.FS32; }
     __device__ static float iv(const PassArgs& a, int i) { return plane_at(a.iv32, i); }
+    __device__ static const float* ivp(const PassArgs& a) { return a.iv32; }
+    __device__ static const float* ring_geo(const PassArgs& a) { return a.hgeo32; }
+    __device__ static const float* ring_iv(const PassArgs& a) { return a.hiv32; }
 };
 
 // Tile staging: the block's cells [tb, tb + tc) have their state in LDS
