@@ -96,14 +96,16 @@ __device__ __forceinline__ void load_ring_in(const PassArgs& a, int e, CellIn<T,
 
 // Pass A of one staged cell (local index l): gradient, limiter and the face
 // state of each of its faces into LDS (a tile cell also writes the ghost side
-// of its boundary faces).  lnb_out receives the local neighbour indices.
+// of its boundary faces).  lnb_out receives the local neighbour indices and,
+// in FP32-mixed, dnb_out the rounded differences u_nb - u of the neighbour
+// states (pass B forms the face's neighbour-side increment from the same values).
 // Empty slots carry zero geometry (the packing writes zeros), so every sum and
 // bound is exactly that of the two-pass kernels, which skip them.
 template <typename T, int MAXDEG>
 __device__ __forceinline__ void fused_cell_a(const PassArgs& a, const TileLists& L, const CellIn<T, MAXDEG>& c, int l,
                                              bool own, const double2* __restrict__ sU, int su_pitch,
                                              typename FusedTypes<T>::V* __restrict__ sFS, int fs_pitch,
-                                             int lnb_out[MAXDEG], bool& bad) {
+                                             int lnb_out[MAXDEG], T dnb_out[MAXDEG][NV], bool& bad) {
     using FT = FusedTypes<T>;
     // FP64: the state itself; FP32-mixed: differences from it (u = 0, see
     // rusanov_delta), formed from the FP64 staged states
@@ -138,7 +140,10 @@ __device__ __forceinline__ void fused_cell_a(const PassArgs& a, const TileLists&
         FusedTypes<double>::get(sU, su_pitch, q, und);
         T un[NV];
 #pragma unroll
-        for (int v = 0; v < NV; v++) un[v] = DELTA ? (T)(und[v] - ud[v]) : (T)und[v];
+        for (int v = 0; v < NV; v++) {
+            un[v] = DELTA ? (T)(und[v] - ud[v]) : (T)und[v];
+            dnb_out[k][v] = un[v];
+        }
         const T If = c.If[k];
         const T omI = T(1) - If;
         const bool isR = (e & LT_SIDE_R) != 0;
@@ -396,16 +401,22 @@ k_fused_step(
         // pass A: the tile cells, then the halo ring (the last lanes take it)
         bool bad = false;
         int lnb[MAXDEG];
+        T dnb[MAXDEG][NV];  // FP32-mixed: u_nb - u of each slot, kept for pass B
 #pragma unroll
-        for (int k = 0; k < MAXDEG; k++) lnb[k] = -1;
-        if (hasT) fused_cell_a<T, MAXDEG>(a, L, ci, tid, true, sU, su_pitch, sFS, fs_pitch, lnb, bad);
+        for (int k = 0; k < MAXDEG; k++) {
+            lnb[k] = -1;
+#pragma unroll
+            for (int v = 0; v < NV; v++) dnb[k][v] = T(0);
+        }
+        if (hasT) fused_cell_a<T, MAXDEG>(a, L, ci, tid, true, sU, su_pitch, sFS, fs_pitch, lnb, dnb, bad);
         for (int q = qh; q < L.nh1; q += FUSED_TILE) {
             if (!early_ring || q != qh) {
                 load_ring_in<T, MAXDEG>(a, h0 + q, ch);
                 load_local(a.lh, a.nh1tot, h0 + q, ch);
             }
             int ln[MAXDEG];
-            fused_cell_a<T, MAXDEG>(a, L, ch, L.tc + q, false, sU, su_pitch, sFS, fs_pitch, ln, bad);
+            T dn[MAXDEG][NV];
+            fused_cell_a<T, MAXDEG>(a, L, ch, L.tc + q, false, sU, su_pitch, sFS, fs_pitch, ln, dn, bad);
         }
         __syncthreads();
         // pass B over the tile
@@ -447,10 +458,8 @@ k_fused_step(
                     } else {
                         FT::get(sFS + (size_t)((tw < MAXDEG ? tw : 0) * FT::PER) * fs_pitch, fs_pitch, q, Un);
                     }
-                    double und[NV];
-                    FusedTypes<double>::get(sU, su_pitch, q, und);
 #pragma unroll
-                    for (int v = 0; v < NV; v++) Un[v] += (float)(und[v] - uown[v]);
+                    for (int v = 0; v < NV; v++) Un[v] += dnb[k][v];  // (float)(u_nb - u), as pass A rounded it
                     rusanov_delta((float)a.gamma, fb, Uo, Un, ci.n0[k], ci.n1[k], F);
                 } else {
                     FT::get(sFS + (size_t)((tw < MAXDEG ? tw : 0) * FT::PER) * fs_pitch, fs_pitch, q, Un);
