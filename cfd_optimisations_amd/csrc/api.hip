@@ -138,14 +138,13 @@ struct hcfd_ctx {
 #ifndef HCFD_LDS_GRANULE
 #define HCFD_LDS_GRANULE 2048
 #endif
-// Blocks per CU the single-kernel step's LDS budget is sized for: FP32-mixed on
-// triangles (five, the waves per SIMD its register budget allows; four on other
-// cells), and FP64 on cells of at most four faces (octagon meshes: two).
+// Blocks per CU the single-kernel step's LDS budget is sized for, the waves per
+// SIMD its register budget allows on triangles: FP32-mixed five, FP64 four.
 #ifndef HCFD_FUSED_MIXED_BLOCKS
 #define HCFD_FUSED_MIXED_BLOCKS 5
 #endif
 #ifndef HCFD_FUSED_FP64_BLOCKS
-#define HCFD_FUSED_FP64_BLOCKS 3
+#define HCFD_FUSED_FP64_BLOCKS 4
 #endif
 
 namespace {
@@ -335,17 +334,18 @@ hcfd_ctx* create_ctx(int n, const hcfd_mesh_view* views, int precision, int flag
     HIPCHK(hipMemsetAsync(c->bctab, 0, sizeof(BCEntry) * c->nbctab, c->st));
     c->sched.xcd_remap = 1;
     // halo tiles of the single-kernel step; used when their LDS footprint fits
-    // LDS share of one block at the occupancy the registers allow: FP64 three
-    // blocks of four wavefronts per CU (two for octagon meshes), FP32-mixed five
-    // on triangles (four otherwise)
+    // LDS share of one block at the occupancy the registers allow: FP64 four
+    // blocks of four wavefronts per CU on triangles (three on quadrilaterals, two
+    // on octagon meshes), FP32-mixed five on triangles (four otherwise)
     {
         const int blocks = precision == HCFD_MIXED ? (c->deg_bucket <= 3 ? HCFD_FUSED_MIXED_BLOCKS : 4)
-                                                   : (c->deg_bucket <= 4 ? HCFD_FUSED_FP64_BLOCKS : 2);
+                                                   : (c->deg_bucket <= 3 ? HCFD_FUSED_FP64_BLOCKS
+                                                                         : c->deg_bucket <= 4 ? 3 : 2);
         // the share is rounded down to whole HCFD_LDS_GRANULE units: measured, blocks
         // of 53.9 KB (under a third of 160 KB) run two per CU, blocks of 53.5 KB three
         const size_t budget = (size_t)160 * 1024 / blocks / HCFD_LDS_GRANULE * HCFD_LDS_GRANULE;
-        build_halo_tables(c->Nint, c->NcPad, c->deg_bucket, c->m.cnb, c->m.cside, 256, precision == HCFD_MIXED, budget,
-                          c->halo, c->st);
+        build_halo_tables(c->Nint, c->NcPad, c->deg_bucket, c->m.cnb, c->m.cside,
+                          fused_cells(precision == HCFD_MIXED), precision == HCFD_MIXED, budget, c->halo, c->st);
     }
     c->fused_lds = fused_lds_bytes(precision == HCFD_MIXED, c->deg_bucket, c->halo.smax, c->halo.fsmax);
     {
@@ -446,7 +446,7 @@ Launch make_launch(hcfd_ctx* c, bool cfl, int cur, int pass) {
     a.Nc = c->Nc;
     a.NcPad = c->NcPad;
     a.ninst = c->ninst;
-    a.tile_cells = fused ? 256 : s.block / L.v.lpc;
+    a.tile_cells = fused ? fused_cells(c->precision == HCFD_MIXED) : s.block / L.v.lpc;
     a.ntiles = fused ? c->halo.ntiles : (c->Nint + a.tile_cells - 1) / a.tile_cells;
     a.tile_start = fused ? c->halo.tile_start : nullptr;
     a.xcd_remap = s.xcd_remap;

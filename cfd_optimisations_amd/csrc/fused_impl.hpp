@@ -1,8 +1,11 @@
 // fused_impl.hpp -- the single-kernel step over halo tiles (included by
 // fused_fp64.hip and fused_mixed.hip).
 //
-// One block owns one tile T of 256 consecutive interior cells and performs a
-// whole step of Euler::run (src/Base/Euler.cpp:85-97) for it:
+// One block of four wavefronts owns one tile T of consecutive interior cells
+// and performs a whole step of Euler::run (src/Base/Euler.cpp:85-97) for it.
+// FP64: at most FUSED_CELLS (192) cells, one per lane of three wavefronts, the
+// fourth computing the tile's halo ring; FP32-mixed: at most 256 cells, one per
+// lane, the last lanes taking the ring in extra rounds (fused_lanes).  Steps:
 //   1. stage in LDS the states U^n of T, of its halo ring H1 and of their
 //      other neighbours S2 (halo.hpp);
 //   2. for every cell of T and H1: Green-Gauss gradient (GaussGreen.cpp:14-40),
@@ -30,7 +33,7 @@
 
 namespace hcfd {
 
-constexpr int FUSED_TILE = 256;
+constexpr int FUSED_TILE = 256;  // threads per block: four wavefronts (tile cells: fused_cells, halo.hpp)
 
 // LDS element types: FP64 cell state / face state = two double2; FP32 = one float4.
 template <typename T>
@@ -265,11 +268,12 @@ struct TileMin {
 // inst: the lane's cell's instance (-1 without a cell); bf, bl: the instances of
 // the tile's first and last cell.  A tile inside one instance (bf == bl, the
 // common case) needs only the first partial.
-template <bool MULTI>
+// NWAVES: the wavefronts that arrive (those that carry tile cells).
+template <bool MULTI, int NWAVES>
 __device__ __forceinline__ void cfl_tile_lds(const PassArgs& a, int tile, int inst, int bf, int bl, double rate,
                                              TileMin* tm) {
     const int lane = threadIdx.x & (WAVE - 1);
-    const int nwaves = FUSED_TILE / WAVE;
+    const int nwaves = NWAVES;
     double vf = rate, vl = 0.0;
     const bool two = MULTI && bf != bl;  // block-uniform
     if (two) {
@@ -296,16 +300,46 @@ __device__ __forceinline__ void cfl_tile_lds(const PassArgs& a, int tile, int in
 }
 
 // Minimum waves per SIMD for the single-kernel step on triangles (bounds its
-// VGPRs; the LDS of a tile allows three FP64 blocks of four waves per CU and
-// five FP32-mixed ones; cells of more faces two).  FP32-mixed at five waves
-// spills 16-40 B per lane and is still 9% faster on the 1,024-wedge batch than
-// at four (measured 7.49 against 8.25 ms), neutral on the 16M-triangle ramp.
+// VGPRs; the LDS of a tile allows HCFD_FUSED_FP64_BLOCKS FP64 blocks of four
+// waves per CU and HCFD_FUSED_MIXED_BLOCKS FP32-mixed ones; cells of more faces
+// two).  FP64 at four waves: 118 VGPRs without spills, the ring wavefront
+// holding no tile cell's inputs.  FP32-mixed at five: 93.
 #ifndef HCFD_WAVES_FUSED
-#define HCFD_WAVES_FUSED 3
+#define HCFD_WAVES_FUSED 4
 #endif
 #ifndef HCFD_WAVES_FUSED_MIXED
 #define HCFD_WAVES_FUSED_MIXED 5
 #endif
+
+// Lane roles of a block.  RW (FP64): three wavefronts carry the tile's (at most
+// FUSED_CELLS) cells and one computes pass A of the tile's halo ring, so the
+// ring's inputs load during the staging and no lane holds the inputs of a tile
+// cell and a ring cell at once; the ring's wavefront is rotated from tile to
+// tile, so every SIMD gets ring work (pass A only) as well as tile work.
+// Otherwise (FP32-mixed, where the extra pass-B-free wavefront costs more issue
+// than it saves): every lane carries a tile cell and the lanes from the top of
+// a rotated order take the ring cells in extra rounds.
+struct FusedLanes {
+    bool ring;  // RW: this lane's wavefront computes the halo ring
+    int tl;     // the lane's tile cell (RW ring wavefront: its ring lane 0..63)
+    int rq;     // !RW: the lane's first ring entry (none when >= the ring's size)
+};
+template <bool RW>
+__device__ __forceinline__ FusedLanes fused_lanes(int tile) {
+    const int tid = threadIdx.x, w = tid / WAVE, lane = tid & (WAVE - 1);
+    const int rw = (int)((blockIdx.x + tile) & (FUSED_TILE / WAVE - 1));
+    FusedLanes f;
+    if constexpr (RW) {
+        f.ring = w == rw;
+        f.tl = f.ring ? lane : (w < rw ? w : w - 1) * WAVE + lane;
+        f.rq = FUSED_TILE;
+    } else {
+        f.ring = false;
+        f.tl = tid;
+        f.rq = FUSED_TILE - 1 - ((tid + WAVE * rw) & (FUSED_TILE - 1));
+    }
+    return f;
+}
 
 // LOOP: a block walks several tiles (a persistent grid); otherwise each block
 // owns at most one tile, and the kernel holds no loop, so its register budget
@@ -331,6 +365,7 @@ k_fused_step(
         tmin->arrived = 0;
     }  // first read after the tile loop's first barrier
 
+    constexpr bool RW = sizeof(T) == 8;  // a ring wavefront (fused_lanes)
     advance_clocks(a);  // dt of this step: fixed, or reduced from the CFL bound of U^n
     int tfirst, step, tend;
     tile_walk(a, tfirst, step, tend);
@@ -345,33 +380,32 @@ k_fused_step(
         L.Nint = a.Nint;
         L.sH1 = nullptr;  // neighbours' local indices come from lt / lh
         L.sS2 = nullptr;
-        const int i = L.tb + tid;
-        const bool hasT = tid < L.tc;
+        const FusedLanes fl = fused_lanes<RW>(tile);
+        const int tl = fl.tl;
+        const int i = L.tb + tl;
+        const bool hasT = !fl.ring && tl < L.tc;
         // instances of the tile's first and last cell (batches)
         int bf = 0, bl = 0;
         if constexpr (MULTI) {
             bf = instance_from(a.inst_off, a.chunk_inst, L.tb);
             bl = instance_from(a.inst_off, a.chunk_inst, L.tb + L.tc - 1);
         }
-        // this lane's tile cell: its inputs are in flight while the tile is staged
-        CellIn<T, MAXDEG> ci;
+        // the lane's tile cell, or the ring wavefront's first ring cell: its
+        // inputs are in flight while the tile is staged; without a ring
+        // wavefront, so are those of the first ring cell of the lanes that take one
+        CellIn<T, MAXDEG> ci, ch;
         double uown[NV] = {0, 0, 0, 0};  // U^n of the lane's tile cell (read back from LDS in pass B)
         if (hasT) {
             load_cell_in<T, MAXDEG>(a, i, ci);
             load_local(a.lt, a.NcPad, i, ci);
             load_cell<W, NV>(a.U, i, uown);
+        } else if (RW && fl.ring && tl < L.nh1) {
+            load_ring_in<T, MAXDEG>(a, h0 + tl, ci);
+            load_local(a.lh, a.nh1tot, h0 + tl, ci);
         }
-        // the halo ring goes to one wavefront, rotated from tile to tile so the
-        // extra round does not always land on the same SIMD; in FP32 (where the
-        // registers allow it without spilling) the inputs of its first ring cell
-        // are in flight during the staging as well
-        constexpr bool early_ring = sizeof(T) == 4;
-        const int rtid = (tid + WAVE * ((blockIdx.x + tile) & (FUSED_TILE / WAVE - 1))) & (FUSED_TILE - 1);
-        const int qh = FUSED_TILE - 1 - rtid;
-        CellIn<T, MAXDEG> ch;
-        if (early_ring && qh < L.nh1) {
-            load_ring_in<T, MAXDEG>(a, h0 + qh, ch);
-            load_local(a.lh, a.nh1tot, h0 + qh, ch);
+        if (!RW && fl.rq < L.nh1) {
+            load_ring_in<T, MAXDEG>(a, h0 + fl.rq, ch);
+            load_local(a.lh, a.nh1tot, h0 + fl.rq, ch);
         }
         // staged halo cell q = tid (ring H1, then S2): its id and state are in
         // flight before the barrier too; further rounds (more than 256 staged
@@ -384,7 +418,7 @@ k_fused_step(
             load_cell<W, NV>(a.U, gq, uq);
         }
         __syncthreads();  // the previous tile's readers are done
-        if (hasT) FusedTypes<double>::put(sU, su_pitch, tid, uown);
+        if (hasT) FusedTypes<double>::put(sU, su_pitch, tl, uown);
         for (int q = tid; q < nhalo; q += FUSED_TILE) {
             int g = gq;
             double ud[NV];
@@ -398,7 +432,7 @@ k_fused_step(
             FusedTypes<double>::put(sU, su_pitch, L.tc + q, ud);
         }
         __syncthreads();
-        // pass A: the tile cells, then the halo ring (the last lanes take it)
+        // pass A: the tile cells and the halo ring
         bool bad = false;
         int lnb[MAXDEG];
         T dnb[MAXDEG][NV];  // FP32-mixed: u_nb - u of each slot, kept for pass B
@@ -408,17 +442,35 @@ k_fused_step(
 #pragma unroll
             for (int v = 0; v < NV; v++) dnb[k][v] = T(0);
         }
-        if (hasT) fused_cell_a<T, MAXDEG>(a, L, ci, tid, true, sU, su_pitch, sFS, fs_pitch, lnb, dnb, bad);
-        for (int q = qh; q < L.nh1; q += FUSED_TILE) {
-            if (!early_ring || q != qh) {
-                load_ring_in<T, MAXDEG>(a, h0 + q, ch);
-                load_local(a.lh, a.nh1tot, h0 + q, ch);
+        if (hasT) fused_cell_a<T, MAXDEG>(a, L, ci, tl, true, sU, su_pitch, sFS, fs_pitch, lnb, dnb, bad);
+        if constexpr (RW) {
+            if (fl.ring) {
+                for (int q = tl; q < L.nh1; q += WAVE) {
+                    if (q != tl) {
+                        load_ring_in<T, MAXDEG>(a, h0 + q, ci);
+                        load_local(a.lh, a.nh1tot, h0 + q, ci);
+                    }
+                    int ln[MAXDEG];
+                    T dn[MAXDEG][NV];
+                    fused_cell_a<T, MAXDEG>(a, L, ci, L.tc + q, false, sU, su_pitch, sFS, fs_pitch, ln, dn, bad);
+                }
             }
-            int ln[MAXDEG];
-            T dn[MAXDEG][NV];
-            fused_cell_a<T, MAXDEG>(a, L, ch, L.tc + q, false, sU, su_pitch, sFS, fs_pitch, ln, dn, bad);
+        } else {
+            for (int q = fl.rq; q < L.nh1; q += FUSED_TILE) {
+                if (q != fl.rq) {
+                    load_ring_in<T, MAXDEG>(a, h0 + q, ch);
+                    load_local(a.lh, a.nh1tot, h0 + q, ch);
+                }
+                int ln[MAXDEG];
+                T dn[MAXDEG][NV];
+                fused_cell_a<T, MAXDEG>(a, L, ch, L.tc + q, false, sU, su_pitch, sFS, fs_pitch, ln, dn, bad);
+            }
         }
         __syncthreads();
+        if (RW && fl.ring) {  // no pass B on the ring wavefront
+            if (bad) a.fault[0] = 1;
+            return;
+        }
         // pass B over the tile
         double rate = 0.0;
         int lane_inst = -1;
@@ -431,7 +483,7 @@ k_fused_step(
             const double dt = a.dt[inst];
             double R[NV] = {0.0, 0.0, 0.0, 0.0};
             // U^n of the cell back from LDS (not held in registers through pass A)
-            FusedTypes<double>::get(sU, su_pitch, tid, uown);
+            FusedTypes<double>::get(sU, su_pitch, tl, uown);
             FluxBase fb;
             if constexpr (sizeof(T) == 4) fb = flux_base((float)a.gamma, uown);
             unsigned gmask = 0, present = 0;
@@ -445,10 +497,10 @@ k_fused_step(
                 int q = e & LT_LOCAL_MASK;
                 if (q >= fs_pitch || tw >= MAXDEG) {
                     bad = true;
-                    q = tid;
+                    q = tl;
                 }
                 T Uo[NV], Un[NV];
-                FT::get(sFS + (size_t)(k * FT::PER) * fs_pitch, fs_pitch, tid, Uo);
+                FT::get(sFS + (size_t)(k * FT::PER) * fs_pitch, fs_pitch, tl, Uo);
                 T F[NV];
                 if constexpr (sizeof(T) == 4) {
                     // increments of the two face states (see pass_b_cell_mixed)
@@ -505,7 +557,7 @@ k_fused_step(
         } else if (bad) {
             a.fault[0] = 1;
         }
-        if constexpr (CFL) cfl_tile_lds<MULTI>(a, tile, lane_inst, bf, bl, rate, tmin);
+        if constexpr (CFL) cfl_tile_lds<MULTI, (RW ? 3 : 4)>(a, tile, lane_inst, bf, bl, rate, tmin);
     };
     // One tile per block (the default grid) runs outside any loop: inside one,
     // the compiler hoists the tile cell's LDS addresses out of it and, in FP64,

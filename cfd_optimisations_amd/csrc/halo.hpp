@@ -31,6 +31,13 @@ constexpr int LT_TWIN_MASK = 7;
 constexpr int LT_GHOST = 1 << 13;
 constexpr int LT_SIDE_R = 1 << 14;
 
+// Cells per tile of the single-kernel step (blocks of four wavefronts): FP64
+// three wavefronts (the fourth computes the tile's halo ring, about 54 cells on
+// triangles), FP32-mixed all four (the ring, about 61 cells, in extra rounds).
+constexpr int FUSED_CELLS = 192;
+constexpr int FUSED_CELLS_MIXED = 256;
+inline int fused_cells(bool mixed) { return mixed ? FUSED_CELLS_MIXED : FUSED_CELLS; }
+
 struct HaloTables {
     int TC = 0, ntiles = 0;
     int* tile_start = nullptr;  // [ntiles + 1] first cell of each tile (tiles hold at most TC cells)
