@@ -317,13 +317,13 @@ hcfd_ctx* create_ctx(int n, const hcfd_mesh_view* views, int precision, int flag
     const int maxtiles = (c->Nint + 15) / 16 + 1;  // smallest pass-A tile: 64 threads / 4 lanes per cell
     c->pmin_first = dmalloc<double>(maxtiles);
     c->pmin_last = dmalloc<double>(maxtiles);
-    c->dtacc = dmalloc<unsigned long long>(n);
+    c->dtacc = dmalloc<unsigned long long>(n + 1);  // + the one-instance reduction's block ticket
     c->dt = dmalloc<double>(n);
     c->tdev = dmalloc<double>(n);
     c->lastdt = dmalloc<double>(n);
     c->fault = dmalloc<int>(n);
     {
-        HIPCHK(hipMemsetAsync(c->dtacc, 0, sizeof(unsigned long long) * n, c->st));  // maximum rate: 0
+        HIPCHK(hipMemsetAsync(c->dtacc, 0, sizeof(unsigned long long) * (n + 1), c->st));  // maximum rate: 0
         HIPCHK(hipMemsetAsync(c->tdev, 0, sizeof(double) * n, c->st));
         HIPCHK(hipMemsetAsync(c->lastdt, 0, sizeof(double) * n, c->st));
         HIPCHK(hipMemsetAsync(c->fault, 0, sizeof(int) * n, c->st));
@@ -697,7 +697,7 @@ void run_steps_noalloc(hcfd_ctx* c, int nsteps, double dt, double cfl) {
     const long long tiles = c->sched.fused ? (long long)c->halo.ntiles
                                            : (c->Nint + c->sched.block / c->sched.lanes_per_cell - 1) /
                                                  (c->sched.block / c->sched.lanes_per_cell);
-    const size_t need = sizeof(double) * (size_t)(2 * tiles + c->ninst);
+    const size_t need = sizeof(double) * (size_t)(2 * tiles + c->ninst + 1);
     if (need > HCFD_NOALLOC_SCRATCH_BYTES)
         fail(HCFD_ERR_ARG, "hcfd_step_async_noalloc: this context's schedule needs " + std::to_string(need) +
                                " bytes of CFL scratch (" + std::to_string(tiles) + " tiles, " +
@@ -719,9 +719,10 @@ void run_steps_noalloc(hcfd_ctx* c, int nsteps, double dt, double cfl) {
     c->scr_first = base;
     c->scr_last = base + tiles;
     c->scr_acc = reinterpret_cast<unsigned long long*>(base + 2 * tiles);
-    // the per-instance maxima start at zero (the array keeps the last call's contents)
-    const int nb = (c->ninst + 255) / 256;
-    k_zero_u64<<<nb, 256, 0, c->st>>>(c->ninst, c->scr_acc);
+    // the per-instance maxima and the reduction's ticket start at zero (the array
+    // keeps the last call's contents)
+    const int nb = (c->ninst + 256) / 256;
+    k_zero_u64<<<nb, 256, 0, c->st>>>(c->ninst + 1, c->scr_acc);
     HIPCHK(hipGetLastError());
     if (!(cfl > 0)) {
         k_fill_f64<<<nb, 256, 0, c->st>>>(c->ninst, c->dt, dt);

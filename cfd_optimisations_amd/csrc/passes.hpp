@@ -31,7 +31,8 @@ struct PassArgs {
     const int* chunk_inst = nullptr;    // [ceil(Nint/64)] instance of cell 64c
     double* pmin_first = nullptr;       // [ntiles] per-tile CFL partial (maximum rate) of the tile's first instance
     double* pmin_last = nullptr;        // [ntiles] ... of its last instance
-    unsigned long long* dtacc = nullptr;// [ninst] maximum rate (bit pattern) of cells in neither
+    unsigned long long* dtacc = nullptr;// [ninst + 1] maximum rate (bit pattern) of cells in neither
+                                        // (slot ninst: the one-instance dt reduction's block ticket)
     const double* dt = nullptr;         // [ninst] dt of this step
     const BCEntry* bctab = nullptr;
     const int* ghost_bc = nullptr;

@@ -115,8 +115,45 @@ __global__ __launch_bounds__(1024) void k_dt_reduce(PassArgs a, double cfl, doub
     }
 }
 
+// One instance: the tile partials split over the grid's blocks; each folds its
+// maximum into dtacc[0] (device-scope atomic on the rate's bit pattern, which
+// orders like the value), and the last block to arrive (ticket dtacc[1]) forms
+// dt = cfl * (1 / max) and resets both slots for the next step.
+__global__ __launch_bounds__(256) void k_dt_reduce_one(PassArgs a, double cfl, double* __restrict__ dt) {
+    __shared__ double sh[256 / WAVE];
+    const int G = gridDim.x;
+    const int per = (a.ntiles + G - 1) / G;
+    const int t0 = blockIdx.x * per, t1 = min(t0 + per, a.ntiles);
+    double m = 0.0, m1 = 0.0;
+    const int s = blockDim.x;
+    int tile = t0 + (int)threadIdx.x;
+    for (; tile + s < t1; tile += 2 * s) {
+        m = fmax(m, a.pmin_first[tile]);
+        m1 = fmax(m1, a.pmin_first[tile + s]);
+    }
+    if (tile < t1) m = fmax(m, a.pmin_first[tile]);
+    m = block_max(fmax(m, m1), sh);
+    if (threadIdx.x == 0) {
+        atomicMax(&a.dtacc[0], dbits(m));
+        __threadfence();
+        if (atomicAdd(&a.dtacc[1], 1ull) == (unsigned long long)(G - 1)) {
+            __threadfence();
+            const double mx = __longlong_as_double((long long)atomicAdd(&a.dtacc[0], 0ull));
+            dt[0] = cfl * (1.0 / mx);
+            atomicExch(&a.dtacc[0], 0ull);
+            atomicExch(&a.dtacc[1], 0ull);
+        }
+    }
+}
+
 hipError_t launch_dt_reduce(const PassArgs& a, double cfl, double* dt, hipStream_t st) {
-    hipLaunchKernelGGL(k_dt_reduce, dim3(a.ninst), dim3(1024), 0, st, a, cfl, dt);
+    if (a.ninst == 1) {
+        // about 1,024 partials per block, at most 64 blocks
+        const int G = std::max(1, std::min(64, a.ntiles / 1024));
+        hipLaunchKernelGGL(k_dt_reduce_one, dim3(G), dim3(256), 0, st, a, cfl, dt);
+    } else {
+        hipLaunchKernelGGL(k_dt_reduce, dim3(a.ninst), dim3(1024), 0, st, a, cfl, dt);
+    }
     return hipGetLastError();
 }
 

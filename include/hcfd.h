@@ -214,7 +214,7 @@ int hcfd_sync(hcfd_ctx* ctx, double* t);
  * host.  The steps' CFL scratch (per-tile partial maxima and per-instance
  * atomics, which hcfd_step_async keeps in buffers the context allocated) lives
  * in one static 16 MiB device array per GPU, held until the process ends; a
- * context whose schedule needs more (2 x tiles + instances doubles: about a
+ * context whose schedule needs more (2 x tiles + instances + 1 doubles: about a
  * million tiles, some 200M cells in FP64's 192-cell tiles and 260M in
  * FP32-mixed's 256-cell ones) gets HCFD_ERR_ARG.  Launches are eager (no graph is
  * instantiated); a fixed dt is stored by a kernel.  Results equal
