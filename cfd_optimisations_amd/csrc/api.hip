@@ -358,9 +358,11 @@ hcfd_ctx* create_ctx(int n, const hcfd_mesh_view* views, int precision, int flag
         // halo than they update; Hilbert tiles keep the halo near a quarter
         c->fused_default = c->fused_ok && c->halo.split_tiles * 20 <= c->halo.ntiles;
     }
-    if (c->fused_ok)
+    if (c->fused_ok) {
         build_ring_inputs(c->deg_bucket, (long long)P, precision == HCFD_MIXED ? nullptr : c->m.cgeo, c->m.iv,
                           c->cgeo32, c->iv32, c->halo, c->st);
+        if (n > 1) build_instance_tiles(c->m.inst_int_off, n, c->halo, c->st);
+    }
     c->sched.fused = c->fused_default ? 1 : 0;
     HIPCHK(hipStreamSynchronize(c->st));
     return c.release();
@@ -449,6 +451,7 @@ Launch make_launch(hcfd_ctx* c, bool cfl, int cur, int pass) {
     a.tile_cells = fused ? fused_cells(c->precision == HCFD_MIXED) : s.block / L.v.lpc;
     a.ntiles = fused ? c->halo.ntiles : (c->Nint + a.tile_cells - 1) / a.tile_cells;
     a.tile_start = fused ? c->halo.tile_start : nullptr;
+    a.inst_tiles = fused ? c->halo.inst_tiles : nullptr;
     a.xcd_remap = s.xcd_remap;
     a.tiles_per_xcd = (a.ntiles + NXCD - 1) / NXCD;
     a.gamma = c->gamma;

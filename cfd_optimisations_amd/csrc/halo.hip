@@ -356,6 +356,37 @@ __global__ void k_ring_inputs(int nslot, long long nh1, const int* __restrict__ 
 
 }  // namespace
 
+namespace {
+
+// Tile holding cell c: the last tile whose first cell is <= c.
+__device__ int tile_of(const int* __restrict__ tile_start, int ntiles, int c) {
+    int lo = 0, hi = ntiles - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tile_start[mid] <= c) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void k_inst_tiles(const int* __restrict__ inst_off, int ninst, const int* __restrict__ tile_start,
+                             int ntiles, int2* __restrict__ out) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= ninst) return;
+    const int c0 = inst_off[b], c1 = inst_off[b + 1];
+    out[b] = c1 > c0 ? make_int2(tile_of(tile_start, ntiles, c0), tile_of(tile_start, ntiles, c1 - 1))
+                     : make_int2(0, -1);
+}
+
+}  // namespace
+
+void build_instance_tiles(const int* inst_off, int ninst, HaloTables& t, hipStream_t st) {
+    if (t.inst_tiles) (void)hipFree(t.inst_tiles);
+    t.inst_tiles = dalloc<int2>((size_t)std::max(ninst, 1));
+    if (ninst > 0 && t.ntiles > 0)
+        k_inst_tiles<<<nblk(ninst), 256, 0, st>>>(inst_off, ninst, t.tile_start, t.ntiles, t.inst_tiles);
+    HIPCHK(hipGetLastError());
+}
+
 void build_ring_inputs(int nslot, long long P, const double* cgeo, const double* iv, const float* cgeo32,
                        const float* iv32, HaloTables& t, hipStream_t st) {
     (void)P;
@@ -377,7 +408,8 @@ void build_ring_inputs(int nslot, long long P, const double* cgeo, const double*
 
 void free_halo_tables(HaloTables& t) {
     for (void* p : {(void*)t.tile_start, (void*)t.h1_off, (void*)t.h1, (void*)t.s_off, (void*)t.s_ngt, (void*)t.s2,
-                    (void*)t.lt, (void*)t.lh, (void*)t.hgeo, (void*)t.hiv, (void*)t.hgeo32, (void*)t.hiv32})
+                    (void*)t.lt, (void*)t.lh, (void*)t.hgeo, (void*)t.hiv, (void*)t.hgeo32, (void*)t.hiv32,
+                    (void*)t.inst_tiles})
         if (p) (void)hipFree(p);
     t = HaloTables{};
 }

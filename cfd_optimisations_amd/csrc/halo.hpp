@@ -64,6 +64,7 @@ struct HaloTables {
     double* hiv = nullptr;
     float* hgeo32 = nullptr;
     float* hiv32 = nullptr;
+    int2* inst_tiles = nullptr;  // [ninst] first and last tile holding a cell of each instance
 };
 
 // The lists of one tile as the kernels see them, and the local index of cell j
@@ -108,6 +109,10 @@ void build_halo_tables(int Nint, int NcPad, int nslot, const int* cnb, const int
                        size_t lds_budget,
                        HaloTables& out, hipStream_t st);
 void free_halo_tables(HaloTables& t);
+
+// The first and last tile of each instance's interior range [inst_off[b],
+// inst_off[b + 1]) into inst_tiles (batches: the CFL reduction's tile ranges).
+void build_instance_tiles(const int* inst_off, int ninst, HaloTables& t, hipStream_t st);
 
 // Copies the inputs of every H1 entry into hgeo/hiv (FP64 planes cgeo, iv) or
 // hgeo32/hiv32 (FP32-mixed planes cgeo32, iv32: pass null cgeo) from the

@@ -39,6 +39,7 @@ struct PassArgs {
     int* fault = nullptr;               // [ninst]
     // single-kernel step (halo tiles, see halo.hpp)
     const int* tile_start = nullptr;    // [ntiles+1] variable tiles (single-kernel step, CFL bound), else null
+    const int2* inst_tiles = nullptr;   // [ninst] first and last tile of each instance (variable tiles, batches)
     const int* h1_off = nullptr;
     const int* h1 = nullptr;
     const int* s_off = nullptr;

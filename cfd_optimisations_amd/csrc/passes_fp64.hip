@@ -58,7 +58,8 @@ __global__ __launch_bounds__(1024) void k_dt_reduce(PassArgs a, double cfl, doub
     double m = 0.0;
     if (c1 > c0) {
         // tiles of tile_cells cells, or the variable tiles of tile_start (one
-        // instance: every tile; several: thread 0 searches and shares the range)
+        // instance: every tile; several: the range built with the tiles, else
+        // thread 0 searches and shares it)
         __shared__ int s_t[2];
         int t0, t1;
         if (!a.tile_start) {
@@ -67,6 +68,10 @@ __global__ __launch_bounds__(1024) void k_dt_reduce(PassArgs a, double cfl, doub
         } else if (a.ninst == 1) {
             t0 = 0;
             t1 = a.ntiles - 1;
+        } else if (a.inst_tiles) {
+            const int2 r = a.inst_tiles[b];
+            t0 = r.x;
+            t1 = r.y;
         } else {
             if (threadIdx.x == 0) {
                 for (int e = 0; e < 2; e++) {
@@ -97,7 +102,13 @@ __global__ __launch_bounds__(1024) void k_dt_reduce(PassArgs a, double cfl, doub
             for (; tile <= t1; tile += s) m = fmax(m, a.pmin_first[tile]);
             m = fmax(fmax(m, m1), fmax(m2, m3));
         } else {
+            // tiles strictly inside the range start in this instance (their first
+            // partial); the range's two end tiles may start or end in another
             for (int tile = t0 + (int)threadIdx.x; tile <= t1; tile += blockDim.x) {
+                if (tile > t0 && tile < t1) {
+                    m = fmax(m, a.pmin_first[tile]);
+                    continue;
+                }
                 const int f0 = a.tile_start ? a.tile_start[tile] : tile * a.tile_cells;
                 const int l0 = a.tile_start ? a.tile_start[tile + 1] - 1 : min(f0 + a.tile_cells, a.Nint) - 1;
                 const int bf = instance_from(a.inst_off, a.chunk_inst, f0);
