@@ -34,8 +34,10 @@ def pair(case_fn, sched, steps, **kw):
 
 @pytest.mark.parametrize("sched", [TWO_PASS, FUSED], ids=["two_pass", "fused"])
 def test_noalloc_step_equals_step(sched):
-    print(f"no-alloc step: Riemann 120x60 then 24x12, schedule {sched}", flush=True)
-    for nx, ny in ((120, 60), (24, 12)):
+    # 577,600 triangles: over 2,048 CFL tiles in both step paths, so the one-instance
+    # dt reduction runs on two blocks and keeps its block ticket in the static array
+    print(f"no-alloc step: Riemann 760x380 then 24x12, schedule {sched}", flush=True)
+    for nx, ny in ((760, 380), (24, 12)):
         (Ua, ta, da, fa), (Ub, tb, db, fb) = pair(
             lambda: C.make_context(C.riemann_case(nx, ny, seed=2), L.FP64, L.FLAG_ORDER_SFC), sched, 30, cfl=0.5)
         assert np.array_equal(Ua, Ub) and ta == tb and da == db and fa == fb == 0, (nx, ny)
