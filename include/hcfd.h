@@ -114,7 +114,8 @@ typedef struct hcfd_info {
     int fused_available;  /* the halo tiles of this mesh fit in LDS and their local-index tables
                              were built (see hcfd_schedule.fused); the
                              single-kernel step is the default schedule when, in addition,
-                             at most 5% of the 256-cell tiles had to be split to fit */
+                             at most 5% of the tiles (192 cells in FP64, 256 in FP32-mixed)
+                             had to be split to fit */
     int fused_lds_bytes;  /* LDS per block of the single-kernel step */
     int halo_max_cells;   /* most cells one tile stages: its own (at most 256) + halo ring + their neighbours */
 } hcfd_info;
