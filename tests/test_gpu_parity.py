@@ -292,6 +292,43 @@ def test_schedules_bitwise_identical():
             assert np.array_equal(U, ref), sched
 
 
+@pytest.mark.parametrize("prec", [L.FP64, L.MIXED], ids=["fp64", "mixed"])
+def test_persistent_single_kernel_loops(prec):
+    """The single kernel's persistent-grid variants, whose blocks walk several tiles
+    (one grid block per CU, over 350 tiles): equal to the one-tile-per-block launch and
+    to the two-pass path, bit for bit, on a single mesh and on a batch, with and
+    without the XCD remap."""
+    print(f"persistent single kernel: precision {prec}", flush=True)
+    loops = [dict(FUSED, persistent=1, xcd_remap=1), dict(FUSED, persistent=1, xcd_remap=0)]
+    case = C.ramp_case(300, 150, 3.0, 20.0, seed=0)  # 90,000 triangles
+    ref = None
+    for sched in [TWO_PASS, FUSED] + loops:
+        with C.make_context(case, prec, L.FLAG_ORDER_SFC) as ctx:
+            ctx.set_schedule(**sched)
+            t = ctx.step(25, cfl=0.5)
+            out = (ctx.get_state(), t, ctx.fault())
+        if ref is None:
+            ref = out
+            assert ref[2] == 0
+        else:
+            assert np.array_equal(out[0], ref[0]) and out[1] == ref[1] and out[2] == 0, sched
+    cs = [C.ramp_case(200, 100, 2.5, 15.0, seed=1), C.ramp_case(40, 20, 3.0, 20.0, seed=2),
+          C.ramp_case(180, 90, 1.8, 8.0, seed=3)]
+    ref = None
+    for sched in [FUSED] + loops:
+        with C.make_batch(cs, prec, L.FLAG_ORDER_SFC) as b:
+            b.set_schedule(**sched)
+            b.step(25, cfl=0.5)
+            out = ([b.inst_state(k) for k in range(len(cs))], b.status())
+        if ref is None:
+            ref = out
+        else:
+            for k in range(len(cs)):
+                assert np.array_equal(out[0][k], ref[0][k]), (sched, k)
+            for x, y in zip(out[1], ref[1]):
+                assert np.array_equal(x, y), sched
+
+
 @pytest.mark.parametrize("sched", [TWO_PASS, FUSED], ids=["two_pass", "fused"])
 def test_graph_replay_matches_eager(sched):
     case = C.riemann_case(40, 40, seed=2)
