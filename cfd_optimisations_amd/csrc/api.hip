@@ -1479,6 +1479,139 @@ int hcfd_batch_get_status(hcfd_ctx* c, double* t, double* last_dt, int* fault) {
     });
 }
 
+// ---- checkpoint / resume ------------------------------------------------------
+// File: "HCFDCKP1", int32 version (1), int32 ninst, int64 Nc (all instances,
+// ghosts included), int32 precision, int32 stages, int32 Nc of each instance,
+// float64 clock and last dt of each instance, int32 fault flag of each, then U
+// as four float64 arrays of Nc values in the caller's cell order (instances
+// one after another), then a 64-bit FNV-1a hash of those U bytes.  Host byte
+// order (little-endian on the platforms this library builds for).
+namespace {
+
+constexpr char CKP_MAGIC[8] = {'H', 'C', 'F', 'D', 'C', 'K', 'P', '1'};
+
+struct Fnv1a {
+    unsigned long long h = 1469598103934665603ull;
+    void add(const void* p, size_t n) {
+        const unsigned char* b = static_cast<const unsigned char*>(p);
+        for (size_t i = 0; i < n; i++) h = (h ^ b[i]) * 1099511628211ull;
+    }
+};
+
+struct File {
+    std::FILE* f = nullptr;
+    std::string path;
+    File(const char* p, const char* mode) : f(std::fopen(p, mode)), path(p) {
+        if (!f) fail(HCFD_ERR_IO, std::string("cannot open ") + p);
+    }
+    ~File() {
+        if (f) std::fclose(f);
+    }
+    void write(const void* p, size_t n) {
+        if (n && std::fwrite(p, 1, n, f) != n) fail(HCFD_ERR_IO, path + ": write failed");
+    }
+    void read(void* p, size_t n) {
+        if (n && std::fread(p, 1, n, f) != n) fail(HCFD_ERR_IO, path + ": truncated checkpoint");
+    }
+    void close() {
+        const int r = std::fclose(f);
+        f = nullptr;
+        if (r != 0) fail(HCFD_ERR_IO, path + ": close failed");
+    }
+};
+
+}  // namespace
+
+int hcfd_checkpoint_save(hcfd_ctx* c, const char* path) {
+    return guarded([&] {
+        if (!c || !path) fail(HCFD_ERR_ARG, "null");
+        const int n = c->ninst;
+        std::vector<double> t(n), dt(n);
+        std::vector<int> fault(n), sizes(n);
+        HIPCHK(hipMemcpyAsync(t.data(), c->tdev, sizeof(double) * n, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipMemcpyAsync(dt.data(), c->lastdt, sizeof(double) * n, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(hipMemcpyAsync(fault.data(), c->fault, sizeof(int) * n, hipMemcpyDeviceToHost, c->st));
+        HIPCHK(launch_export(c->sched.aosoa_width, NV, c->Nc, c->d_old_of_new, c->U[c->cur], c->staging, c->Nc, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        for (int k = 0; k < n; k++) sizes[k] = c->inst_cell_off[k + 1] - c->inst_cell_off[k];
+        File f(path, "wb");
+        const int head[2] = {1, n};
+        const long long Nc = c->Nc;
+        const int pre[2] = {c->precision, c->stages};
+        f.write(CKP_MAGIC, sizeof CKP_MAGIC);
+        f.write(head, sizeof head);
+        f.write(&Nc, sizeof Nc);
+        f.write(pre, sizeof pre);
+        f.write(sizes.data(), sizeof(int) * n);
+        f.write(t.data(), sizeof(double) * n);
+        f.write(dt.data(), sizeof(double) * n);
+        f.write(fault.data(), sizeof(int) * n);
+        // one variable at a time through a host buffer of Nc values
+        std::vector<double> h((size_t)c->Nc);
+        Fnv1a hash;
+        for (int v = 0; v < NV; v++) {
+            HIPCHK(hipMemcpy(h.data(), c->staging + (size_t)v * c->Nc, sizeof(double) * c->Nc,
+                             hipMemcpyDeviceToHost));
+            hash.add(h.data(), sizeof(double) * h.size());
+            f.write(h.data(), sizeof(double) * h.size());
+        }
+        f.write(&hash.h, sizeof hash.h);
+        f.close();
+    });
+}
+
+int hcfd_checkpoint_load(hcfd_ctx* c, const char* path) {
+    return guarded([&] {
+        if (!c || !path) fail(HCFD_ERR_ARG, "null");
+        HIPCHK(hipStreamSynchronize(c->st));  // steps still in flight read the state and the staging buffer
+        const int n = c->ninst;
+        File f(path, "rb");
+        char magic[8];
+        int head[2], pre[2];
+        long long Nc = 0;
+        f.read(magic, sizeof magic);
+        if (std::memcmp(magic, CKP_MAGIC, sizeof magic) != 0) fail(HCFD_ERR_IO, f.path + ": not a checkpoint");
+        f.read(head, sizeof head);
+        if (head[0] != 1) fail(HCFD_ERR_IO, f.path + ": checkpoint version " + std::to_string(head[0]));
+        f.read(&Nc, sizeof Nc);
+        f.read(pre, sizeof pre);
+        if (head[1] != n || Nc != c->Nc)
+            fail(HCFD_ERR_ARG, f.path + ": checkpoint of " + std::to_string(head[1]) + " instance(s), " +
+                                   std::to_string(Nc) + " cells; this context has " + std::to_string(n) + ", " +
+                                   std::to_string(c->Nc));
+        std::vector<int> sizes(n), fault(n);
+        std::vector<double> t(n), dt(n);
+        f.read(sizes.data(), sizeof(int) * n);
+        for (int k = 0; k < n; k++)
+            if (sizes[k] != c->inst_cell_off[k + 1] - c->inst_cell_off[k])
+                fail(HCFD_ERR_ARG, f.path + ": instance " + std::to_string(k) + " has " + std::to_string(sizes[k]) +
+                                       " cells in the checkpoint, " +
+                                       std::to_string(c->inst_cell_off[k + 1] - c->inst_cell_off[k]) + " here");
+        f.read(t.data(), sizeof(double) * n);
+        f.read(dt.data(), sizeof(double) * n);
+        f.read(fault.data(), sizeof(int) * n);
+        std::vector<double> h((size_t)c->Nc);
+        Fnv1a hash;
+        for (int v = 0; v < NV; v++) {
+            f.read(h.data(), sizeof(double) * h.size());
+            hash.add(h.data(), sizeof(double) * h.size());
+            HIPCHK(hipMemcpy(c->staging + (size_t)v * c->Nc, h.data(), sizeof(double) * c->Nc,
+                             hipMemcpyHostToDevice));
+        }
+        unsigned long long stored = 0;
+        f.read(&stored, sizeof stored);
+        if (stored != hash.h) fail(HCFD_ERR_IO, f.path + ": checkpoint state does not match its hash");
+        // nothing on the device has changed yet: the state, clocks and flags go in together
+        HIPCHK(launch_import(c->sched.aosoa_width, NV, c->Nc, c->d_old_of_new, c->staging, c->Nc, c->U[c->cur], c->st));
+        HIPCHK(hipMemcpyAsync(c->tdev, t.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->st));
+        HIPCHK(hipMemcpyAsync(c->lastdt, dt.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->st));
+        HIPCHK(hipMemcpyAsync(c->fault, fault.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->st));
+        HIPCHK(hipStreamSynchronize(c->st));
+        c->t_host = t[0];
+        c->bound_valid = false;  // the next CFL step takes its dt from the loaded state
+    });
+}
+
 int hcfd_batch_instance_size(hcfd_ctx* c, int inst, int* Nc, int* Nf) {
     return guarded([&] {
         if (!c || inst < 0 || inst >= c->ninst) fail(HCFD_ERR_ARG, "bad argument");

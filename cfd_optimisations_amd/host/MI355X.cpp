@@ -236,6 +236,15 @@ void Euler::restart(const std::string& hsln_path) {
     check(hcfd_get_state(domain.ctx(), U), "hcfd_get_state");
 }
 
+void Euler::checkpoint(const std::string& path) {
+    check(hcfd_checkpoint_save(domain.ctx(), path.c_str()), "hcfd_checkpoint_save");
+}
+
+void Euler::resume(const std::string& path) {
+    check(hcfd_checkpoint_load(domain.ctx(), path.c_str()), "hcfd_checkpoint_load");
+    check(hcfd_get_state(domain.ctx(), U), "hcfd_get_state");
+}
+
 void Euler::initialise(const double QIC[NVARS]) {
     check(hcfd_initialise(domain.ctx(), QIC), "hcfd_initialise");
     check(hcfd_get_state(domain.ctx(), U), "hcfd_get_state");

@@ -41,6 +41,11 @@ class Euler {
     // Start from a .hsln file written by run() (primitives of all cells) instead
     // of initialise(); the file must belong to this mesh.
     void restart(const std::string& hsln_path);
+    // Checkpoint / resume, no reference counterpart: the conserved state, the
+    // clock and the last dt, bit for bit (hcfd_checkpoint_save / _load); a
+    // later run() continues from the loaded state.
+    void checkpoint(const std::string& path);
+    void resume(const std::string& path);
     // Conserved state copied back from the GPU into U.
     const double* const* state();
 

@@ -2,7 +2,7 @@
 // through the MI355X drop-in classes:
 //
 //   hcfd_run [mesh.hmsh] [output stem] [steps] [--stages] [--outputs K] [--rk S]
-//            [--restart file.hsln] [--mixed]
+//            [--restart file.hsln] [--resume file.ckp] [--checkpoint file.ckp] [--mixed]
 //   hcfd_run --case case.ini [--dry-run]
 //
 // Default: ../wedge.hmsh, ../sols/MI355X, 1000 steps of dt = 1e-3, writing
@@ -11,7 +11,9 @@
 // (GaussGreen, Limiter, BC copies, Flux, update, BC cells) instead of the
 // fused device step.  --outputs K writes K files after the initial one
 // (dt_out = T / K); --rk S steps with S-stage SSP-RK; --restart starts from a
-// .hsln file instead of the uniform initial state; --mixed builds the Domain
+// .hsln file instead of the uniform initial state; --resume from a checkpoint
+// (bit for bit where the saved run stopped), and --checkpoint writes one after
+// the steps (Euler::checkpoint / resume); --mixed builds the Domain
 // for the FP32-mixed path (MI355X::Domain(mesh, HCFD_MIXED); the per-stage
 // classes of --stages are FP64 only).
 // --case runs the case an INI file describes (CaseFile.hpp): mesh, gamma,
@@ -146,17 +148,21 @@ int main(int argc, char** argv) {
         const int niter = argc > 3 ? std::atoi(argv[3]) : 1000;
         bool stages = false, mixed = false;
         int nout = 1, rk = 1;
-        std::string restart;
+        std::string restart, resume, ckpt;
         for (int k = 4; k < argc; k++) {
             if (std::strcmp(argv[k], "--stages") == 0) stages = true;
             else if (std::strcmp(argv[k], "--outputs") == 0 && k + 1 < argc) nout = std::atoi(argv[++k]);
             else if (std::strcmp(argv[k], "--rk") == 0 && k + 1 < argc) rk = std::atoi(argv[++k]);
             else if (std::strcmp(argv[k], "--restart") == 0 && k + 1 < argc) restart = argv[++k];
+            else if (std::strcmp(argv[k], "--resume") == 0 && k + 1 < argc) resume = argv[++k];
+            else if (std::strcmp(argv[k], "--checkpoint") == 0 && k + 1 < argc) ckpt = argv[++k];
             else if (std::strcmp(argv[k], "--mixed") == 0) mixed = true;
             else throw std::runtime_error(std::string("unknown option ") + argv[k]);
         }
         if (nout < 1) throw std::runtime_error("--outputs must be at least 1");
         if (mixed && stages) throw std::runtime_error("--stages runs the FP64 per-stage classes; drop --mixed");
+        if (stages && (!resume.empty() || !ckpt.empty()))
+            throw std::runtime_error("--resume and --checkpoint go with the device step; drop --stages");
 
         // the case of src/Test.cpp:51-64
         const int nvars = 4;
@@ -179,11 +185,13 @@ int main(int argc, char** argv) {
         intg.gather_BCs(BCmap, QBC);
         intg.initialise(QIC);
         if (!restart.empty()) intg.restart(restart);
+        if (!resume.empty()) intg.resume(resume);
         intg.set_stages(rk);
 
         auto t1 = std::chrono::steady_clock::now();
         if (!stages) {
             intg.run(niter * dt, dt, niter * dt / nout);
+            if (!ckpt.empty()) intg.checkpoint(ckpt);
         } else {
             // Euler::run's loop body (src/Base/Euler.cpp:85-97) stage by stage
             const int Nc = domain.Nc;

@@ -101,6 +101,8 @@ def load() -> ctypes.CDLL:
         "hcfd_stage_bc_cells": [P],
         "hcfd_step": [P, I, D, D, P],
         "hcfd_get_last_dt": [P, P],
+        "hcfd_checkpoint_save": [P, ctypes.c_char_p],
+        "hcfd_checkpoint_load": [P, ctypes.c_char_p],
         "hcfd_get_fault": [P, P],
         "hcfd_time_steps": [P, I, D, D, P],
         "hcfd_set_schedule": [P, P],
@@ -438,6 +440,14 @@ class Context:
         t = D(0.0)
         check(load().hcfd_output_wait(self.h, slot, _rows(Q), ctypes.byref(t)))
         return Q, t.value
+
+    def checkpoint_save(self, path: str) -> None:
+        """Writes the state, clocks, last dt and fault flags (hcfd_checkpoint_save)."""
+        check(load().hcfd_checkpoint_save(self.h, os.fsencode(path)))
+
+    def checkpoint_load(self, path: str) -> None:
+        """Restores a checkpoint of the same mesh(es) (hcfd_checkpoint_load)."""
+        check(load().hcfd_checkpoint_load(self.h, os.fsencode(path)))
 
     def last_dt(self) -> float:
         d = D(0.0)

@@ -233,6 +233,18 @@ int hcfd_step_async_noalloc(hcfd_ctx* ctx, int nsteps, double dt, double cfl);
  * wait before the slot is begun again. */
 int hcfd_output_begin(hcfd_ctx* ctx, int slot);
 int hcfd_output_wait(hcfd_ctx* ctx, int slot, double* const Q[4], double* t);
+/* Checkpoint / resume (no counterpart in the reference, which can restart only
+ * from the primitives of a .hsln file, src/Base/Euler.cpp:115-136, so not bit for
+ * bit).  hcfd_checkpoint_save writes the conserved state of every cell (caller
+ * order, ghosts included) and each instance's clock, last dt and fault flag, in
+ * FP64, with a hash of the state; hcfd_checkpoint_load restores them into a
+ * context built from the same mesh(es) with its BCs set (any precision,
+ * integrator or schedule), after which stepping continues exactly as the saved
+ * run would have.  A checkpoint of other meshes, a truncated file or a hash
+ * mismatch is refused with HCFD_ERR_ARG / HCFD_ERR_IO and leaves the context
+ * unchanged. */
+int hcfd_checkpoint_save(hcfd_ctx* ctx, const char* path);
+int hcfd_checkpoint_load(hcfd_ctx* ctx, const char* path);
 /* dt of the most recent step. */
 int hcfd_get_last_dt(hcfd_ctx* ctx, double* dt);
 /* 1 if any step since the last call produced rho <= 0, p <= 0 or NaN. */

@@ -117,6 +117,23 @@ def test_dropin_ssprk_and_restart(tmp_path, wedge_mesh):
     assert r.returncode == 1 and "cannot open" in r.stderr
 
 
+def test_dropin_checkpoint_resume(tmp_path):
+    """hcfd_run: 100 steps straight through, against 60 steps that end with
+    --checkpoint and 40 more that start with --resume: the final files are equal bit
+    for bit (the reference's t += dt loop takes 60, 40 and 100 steps for these ends)."""
+    w = os.path.join(GOLDEN, "wedge.hmsh")
+    ck = tmp_path / "a.ckp"
+    _run([w, tmp_path / "full", 100])
+    _run([w, tmp_path / "a", 60, "--checkpoint", ck])
+    _run([w, tmp_path / "b", 40, "--resume", ck])
+    _, F = read_hsln(str(tmp_path / "full_1.hsln"))
+    _, B = read_hsln(str(tmp_path / "b_1.hsln"))
+    assert np.array_equal(F, B)
+    r = subprocess.run([RUN, w, str(tmp_path / "x"), "1", "--resume", str(tmp_path / "missing.ckp")],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "cannot open" in r.stderr
+
+
 def test_dropin_mixed_domain_within_bound(tmp_path, wedge_mesh):
     """MI355X::Domain(mesh, HCFD_MIXED) through the drop-in Euler: src/Test.cpp's
     1000 steps stay within the mixed path's stated bound of the oracle (relative

@@ -102,3 +102,73 @@ def test_batch_snapshot_is_concatenated_prims():
             assert np.array_equal(Q[:, off:off + n], od.prims(U5)), k
             off += n
         assert off == Q.shape[1]
+
+
+@pytest.mark.parametrize("prec,stages", [(L.FP64, 1), (L.FP64, 3), (L.MIXED, 1)], ids=["fp64", "fp64_rk3", "mixed"])
+def test_checkpoint_resume_bitexact(tmp_path, prec, stages):
+    """40 CFL steps straight through, against 25 steps, a checkpoint, a fresh
+    context that loads it and 15 more steps: the same state, clock and last dt bit
+    for bit (the resumed run takes its first CFL step from the loaded state)."""
+    print(f"checkpoint/resume: precision {prec}, {stages} stage(s)", flush=True)
+    case = C.riemann_case(60, 30, seed=4, shuffle=True)
+    p = str(tmp_path / "run.ckp")
+    with C.make_context(case, prec, L.FLAG_ORDER_SFC) as a:
+        a.set_integrator(stages)
+        ta = a.step(40, cfl=0.5)
+        Ua, da = a.get_state(), a.last_dt()
+    with C.make_context(case, prec, L.FLAG_ORDER_SFC) as b:
+        b.set_integrator(stages)
+        b.step(25, cfl=0.5)
+        b.checkpoint_save(p)
+    with C.make_context(case, prec, L.FLAG_ORDER_SFC) as c:
+        c.set_integrator(stages)
+        c.checkpoint_load(p)
+        tc = c.step(15, cfl=0.5)
+        assert np.array_equal(c.get_state(), Ua) and tc == ta and c.last_dt() == da and c.fault() == 0
+
+
+@pytest.mark.parametrize("prec", [L.FP64, L.MIXED], ids=["fp64", "mixed"])
+def test_checkpoint_resume_batch(tmp_path, prec):
+    """The same for a batch: every instance's state, clock, last dt and fault flag."""
+    print(f"checkpoint/resume of a batch: precision {prec}", flush=True)
+    cs = [C.ramp_case(40, 20, 3.0, 20.0, seed=1), C.ramp_case(24, 12, 2.0, 10.0, seed=2),
+          C.ramp_case(30, 16, 2.5, 15.0, seed=3)]
+    p = str(tmp_path / "batch.ckp")
+    with C.make_batch(cs, prec, L.FLAG_ORDER_SFC) as a:
+        a.step(30, cfl=0.5)
+        ref = ([a.inst_state(k) for k in range(len(cs))], a.status())
+    with C.make_batch(cs, prec, L.FLAG_ORDER_SFC) as b:
+        b.step(18, cfl=0.5)
+        b.checkpoint_save(p)
+    with C.make_batch(cs, prec, L.FLAG_ORDER_SFC) as c:
+        c.checkpoint_load(p)
+        c.step(12, cfl=0.5)
+        for k in range(len(cs)):
+            assert np.array_equal(c.inst_state(k), ref[0][k]), k
+        for x, y in zip(c.status(), ref[1]):
+            assert np.array_equal(x, y)
+
+
+def test_checkpoint_refuses_other_meshes_and_bad_files(tmp_path):
+    """A checkpoint of another mesh, a truncated file, a file whose state does not
+    match its hash and a missing file are refused, and the context keeps its state."""
+    print("checkpoint refusals", flush=True)
+    p = tmp_path / "a.ckp"
+    with C.make_context(C.riemann_case(20, 10, seed=1), L.FP64, L.FLAG_ORDER_SFC) as a:
+        a.step(3, cfl=0.5)
+        a.checkpoint_save(str(p))
+    with C.make_context(C.riemann_case(22, 10, seed=1), L.FP64, L.FLAG_ORDER_SFC) as b:
+        with pytest.raises(L.HcfdError, match="cells"):
+            b.checkpoint_load(str(p))
+    data = p.read_bytes()
+    (tmp_path / "short.ckp").write_bytes(data[:len(data) // 2])
+    # header: 8 + 8 + 8 + 8 + 4 + 8 + 8 + 4 = 56 bytes for one instance; byte 200 lies in the state
+    (tmp_path / "flip.ckp").write_bytes(data[:200] + bytes([data[200] ^ 1]) + data[201:])
+    (tmp_path / "junk.ckp").write_bytes(b"not a checkpoint" * 8)
+    with C.make_context(C.riemann_case(20, 10, seed=1), L.FP64, L.FLAG_ORDER_SFC) as c:
+        U0 = c.get_state()
+        for name, msg in (("short.ckp", "truncated"), ("flip.ckp", "hash"), ("junk.ckp", "not a checkpoint"),
+                          ("missing.ckp", "cannot open")):
+            with pytest.raises(L.HcfdError, match=msg):
+                c.checkpoint_load(str(tmp_path / name))
+            assert np.array_equal(c.get_state(), U0), name
