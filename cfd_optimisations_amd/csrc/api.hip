@@ -26,12 +26,6 @@
 
 namespace hcfd {
 
-static thread_local std::string g_last_error;
-
-void set_last_error(const std::string& msg) { g_last_error = msg; }
-
-[[noreturn]] void fail(int code, const std::string& msg) { throw Error{code, msg}; }
-
 #define HIPCHK(x)                                                                                   \
     do {                                                                                            \
         hipError_t e_ = (x);                                                                        \
@@ -841,8 +835,6 @@ hcfd_ctx::~hcfd_ctx() {
 // C ABI
 // ============================================================================
 extern "C" {
-
-const char* hcfd_last_error(void) { return g_last_error.c_str(); }
 
 int hcfd_version(void) { return 10000; }
 
