@@ -3,8 +3,8 @@
  * residual (Green-Gauss gradient, min/max limiter, MUSCL + Rusanov flux,
  * ghost-cell boundary conditions, forward-Euler update).
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker that the HIP path is
- * compared against.  Only the test-suite, the smoke entry point and the CPU
+ * TEST INFRASTRUCTURE ONLY.  This file is the CPU restatement that the HIP
+ * path is compared against.  Only the test-suite, the smoke entry point and the CPU
  * baseline leg of the benchmark load it; the product library never links or
  * calls it, and it is never the thing that is measured or shipped.
  *

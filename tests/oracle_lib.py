@@ -1,4 +1,4 @@
-"""ctypes bindings of the checkers: the C oracle (oracle/hcfd_oracle.c) and,
+"""ctypes bindings of the CPU references: the C oracle (oracle/hcfd_oracle.c) and,
 where it was built, the reference's own sources (oracle/_ref).
 
 Test infrastructure: used only by tests, the smoke entry point and the CPU
