@@ -295,6 +295,35 @@ __device__ __forceinline__ double wave_max(double x) {
     return x;
 }
 
+// One step of wave_max_dpp: the maximum of x and the x of the lane that the
+// DPP control CTRL names (a lane without a valid source reads 0, which leaves
+// the maximum of non-negative values unchanged).
+template <int CTRL>
+__device__ __forceinline__ double dpp_max_step(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, true);
+    return fmax(x, __longlong_as_double(((long long)hi << 32) | (unsigned)lo));
+}
+
+// Wave-level maximum over 64 lanes (non-negative values), every lane active:
+// quad permutes [1,0,3,2] and [2,3,0,1] and row rotations by 4 and 8 give each
+// lane the maximum of its row of 16, the row broadcasts of lanes 15 and 31
+// fold the rows into lane 63, whose value is returned to every lane (a
+// wave-uniform, scalar result).  Six steps of two DPP moves and one maximum
+// instead of the shuffle form's six index computations and LDS permutes.
+__device__ __forceinline__ double wave_max_dpp(double x) {
+    x = dpp_max_step<0xB1>(x);   // quad_perm [1,0,3,2]
+    x = dpp_max_step<0x4E>(x);   // quad_perm [2,3,0,1]
+    x = dpp_max_step<0x124>(x);  // row_ror:4
+    x = dpp_max_step<0x128>(x);  // row_ror:8
+    x = dpp_max_step<0x142>(x);  // row_bcast:15
+    x = dpp_max_step<0x143>(x);  // row_bcast:31
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_readlane((int)b, 63), hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 // Block-level maximum; every thread receives the result.  `sh` holds one
 // double per wave of the block.
 __device__ __forceinline__ double block_max(double x, double* sh) {

@@ -283,9 +283,9 @@ __device__ __forceinline__ void cfl_tile_lds(const PassArgs& a, int tile, int in
             else if (inst == bl) vl = rate;
             else atomicMax(&a.dtacc[inst], dbits(rate));
         }
-        vl = wave_max(vl);
+        vl = wave_max_dpp(vl);
     }
-    vf = wave_max(vf);
+    vf = wave_max_dpp(vf);
     if (lane == 0) {
         atomicMax(&tm->first, dbits(vf));
         if (two) atomicMax(&tm->last, dbits(vl));
@@ -534,9 +534,13 @@ k_fused_step(
             if (a.U0) rk_combine<W>(a, i, unew);
             store_cell<W, NV>(a.Unext, i, unew);
             if (nonphysical(a.gamma, unew) || bad) a.fault[inst] = 1;
+            // behind one test of the whole mask, so that the ghost states'
+            // common terms (the divide of the subsonic BCs) are not formed on
+            // cells without a boundary face
+            if (gmask != 0u) {
 #pragma unroll
-            for (int k = 0; k < MAXDEG; k++) {
-                if ((gmask >> k) & 1u) {
+                for (int k = 0; k < MAXDEG; k++) {
+                    if (!((gmask >> k) & 1u)) continue;
                     // the ghost's cell id: its entry among the tile's ghosts at the
                     // head of S2 (staged right after the halo ring)
                     const int gp = (lnb[k] & LT_LOCAL_MASK) - L.tc - L.nh1;
