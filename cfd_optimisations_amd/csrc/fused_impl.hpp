@@ -303,7 +303,7 @@ __device__ __forceinline__ void cfl_tile_lds(const PassArgs& a, int tile, int in
 // VGPRs; the LDS of a tile allows HCFD_FUSED_FP64_BLOCKS FP64 blocks of four
 // waves per CU and HCFD_FUSED_MIXED_BLOCKS FP32-mixed ones; cells of more faces
 // two).  FP64 at four waves: 122 VGPRs without spills, the ring wavefront
-// holding no tile cell's inputs.  FP32-mixed at five: 93.
+// holding no tile cell's inputs.  FP32-mixed at five: 91.
 #ifndef HCFD_WAVES_FUSED
 #define HCFD_WAVES_FUSED 4
 #endif
