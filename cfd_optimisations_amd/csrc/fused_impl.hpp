@@ -303,7 +303,7 @@ __device__ __forceinline__ void cfl_tile_lds(const PassArgs& a, int tile, int in
 // VGPRs; the LDS of a tile allows HCFD_FUSED_FP64_BLOCKS FP64 blocks of four
 // waves per CU and HCFD_FUSED_MIXED_BLOCKS FP32-mixed ones; cells of more faces
 // two).  FP64 at four waves: 122 VGPRs without spills, the ring wavefront
-// holding no tile cell's inputs.  FP32-mixed at five: 91.
+// holding no tile cell's inputs.  FP32-mixed at five: 89.
 #ifndef HCFD_WAVES_FUSED
 #define HCFD_WAVES_FUSED 4
 #endif
@@ -503,18 +503,18 @@ k_fused_step(
                 FT::get(sFS + (size_t)(k * FT::PER) * fs_pitch, fs_pitch, tl, Uo);
                 T F[NV];
                 if constexpr (sizeof(T) == 4) {
-                    // increments of the two face states (see pass_b_cell_mixed)
-                    if (gnb) {
-#pragma unroll
-                        for (int v = 0; v < NV; v++) Un[v] = Uo[v];
-                    } else {
-                        FT::get(sFS + (size_t)((tw < MAXDEG ? tw : 0) * FT::PER) * fs_pitch, fs_pitch, q, Un);
-                    }
+                    // increments of the two face states (see pass_b_cell_mixed);
+                    // a ghost's increment is this cell's own, read again from the
+                    // own slot (one LDS read either way, no branch per face); the
+                    // slot offsets are 24-bit products (slot * PER < 16, fs_pitch <= 1024)
+                    const int ks = gnb ? k : (tw < MAXDEG ? tw : 0);
+                    FT::get(sFS + __umul24((unsigned)(ks * FT::PER), (unsigned)fs_pitch), fs_pitch, gnb ? tl : q, Un);
 #pragma unroll
                     for (int v = 0; v < NV; v++) Un[v] += dnb[k][v];  // (float)(u_nb - u), as pass A rounded it
                     rusanov_delta((float)a.gamma, fb, Uo, Un, ci.n0[k], ci.n1[k], F);
                 } else {
-                    FT::get(sFS + (size_t)((tw < MAXDEG ? tw : 0) * FT::PER) * fs_pitch, fs_pitch, q, Un);
+                    FT::get(sFS + __umul24((unsigned)((tw < MAXDEG ? tw : 0) * FT::PER), (unsigned)fs_pitch), fs_pitch,
+                            q, Un);
                     rusanov((T)a.gamma, Uo, Un, ci.n0[k], ci.n1[k], F);
                 }
 #pragma unroll
