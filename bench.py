@@ -587,7 +587,7 @@ def run_batch(name: str, steps: int, warmup: int, tune: bool, compare_fp64: bool
         "vs_baseline": None, "dtype": "f32+f64" if prec == L.MIXED else "f64", "data": "synthetic",
         "config": {"workload": f"{name}: {w['n']} wedges, Mach U{list(w['mach'])}, angle U{list(w['angle'])} deg, "
                                f"log-U{[int(x) for x in w['tris']]} triangles (seed {w['seed']}), "
-                               f"{'FP32-mixed (FP32 differences, gradients, limiter and flux; compensated residual; FP64 update and dt)' if prec == L.MIXED else 'FP64'}, "
+                               f"{'FP32-mixed (FP32 differences, gradients, limiter and flux; residual accumulated in FP64; FP64 update and dt)' if prec == L.MIXED else 'FP64'}, "
                                f"one launch per step for all instances, hipGraph replay, "
                                f"{'autotuned' if tune else 'default'} schedule, CFL {cfl}; timed steps follow "
                                f"{err_steps} untimed steps from the initial state",

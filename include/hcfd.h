@@ -46,7 +46,8 @@ enum hcfd_status {
 /* Arithmetic of the residual. */
 enum hcfd_precision {
     HCFD_FP64 = 0,  /* plain FP64 path: the drop-in for Base */
-    HCFD_MIXED = 1  /* FP32 state/gradient/limiter/flux, compensated residual, FP64 update and dt */
+    HCFD_MIXED = 1  /* FP32 differences from the FP64 state, gradient, limiter and flux;
+                       residual summed in FP64; FP64 state, update and dt */
 };
 
 /* Same values and order as Base::BCtype (includes/Base/BoundaryConditions.hpp:6-13). */
