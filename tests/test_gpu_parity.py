@@ -557,6 +557,36 @@ def test_mixed_quads_wedge(wedge_mesh):
     assert e[0] < MIXED_RHO_TOL, e
 
 
+def test_c3_mixed_against_fp64_full_size():
+    """The 16M-triangle Mach-3 ramp at full size over its quoted 1,000 CFL steps:
+    the FP32-mixed single kernel against the FP64 path from the same initial state,
+    relative L2 density error within the stated bound and every conserved variable
+    within 1e-4, no fault in either; the mixed single kernel equals the mixed two
+    passes bit for bit over the first 50 steps on the same mesh."""
+    w = C.WORKLOADS["c3_ramp_16m"]
+    case = C.ramp_case(w["nx"], w["ny"], w["mach"], w["angle"], seed=0)
+    with C.make_context(case, L.FP64, L.FLAG_ORDER_SFC) as a:
+        ta = a.step(w["steps"], cfl=w["cfl"])
+        assert a.fault() == 0
+        Ua = a.get_state()
+    with C.make_context(case, L.MIXED, L.FLAG_ORDER_SFC) as m:
+        assert m.schedule()["fused"] == 1
+        tm = m.step(w["steps"], cfl=w["cfl"])
+        assert m.fault() == 0
+        Um = m.get_state()
+    assert abs(tm - ta) / ta < 1e-6
+    e = relerr(Um, Ua)
+    assert e[0] < MIXED_RHO_TOL, e
+    assert e.max() < 1e-4, e
+    del Ua, Um
+    with C.make_context(case, L.MIXED, L.FLAG_ORDER_SFC) as f, \
+            C.make_context(case, L.MIXED, L.FLAG_ORDER_SFC) as t:
+        f.set_schedule(**FUSED)
+        t.set_schedule(**TWO_PASS)
+        assert f.step(50, cfl=w["cfl"]) == t.step(50, cfl=w["cfl"])
+        assert np.array_equal(f.get_state(), t.get_state())
+
+
 @pytest.mark.parametrize("prec", [L.FP64, L.MIXED], ids=["fp64", "mixed"])
 def test_free_stream_preserved_on_16m_ramp(prec):
     """Size-independent check on the 16M-triangle ramp mesh itself: a uniform
