@@ -41,6 +41,14 @@
  *     temporal order (errors falling as h^2 and h^3) on a smooth flow.
  *   SubInlet/SubOutlet/Wall: closed-form ghost states in the tests (the
  *     golden case only exercises Symmetry, SupInlet and SupOutlet).
+ *   orc_initialise: sols/Base_0.hsln.  orc_initialise_prims (the per-cell
+ *     initial state): a uniform state gives orc_initialise's bits, a piecewise
+ *     state gives each interior cell the bits of the uniform form of its value
+ *     and the ghosts the BC update, permuting the state over the interior
+ *     permutes the result exactly, and orc_prims returns it to a few ulps.
+ *   The whole file also runs under AddressSanitizer and
+ *     UndefinedBehaviorSanitizer (tests/test_sanitize.py) with its ordinary
+ *     build's results bit for bit.
  *   orc_rusanov3d / orc_euler_flux3d / orc_prims3d / orc_cons3d (the NDIMS == 3
  *     branches, which no mesh of the reference's format can run): with w = 0
  *     and n_z = 0 they equal the 2-D functions bit for bit (pinned above);
