@@ -132,8 +132,20 @@ int run_case(const std::string& path, bool dry) {
 
 }  // namespace
 
+const char* const USAGE =
+    "usage: hcfd_run [mesh.hmsh] [output stem] [steps] [--stages] [--outputs K] [--rk S]\n"
+    "                [--restart file.hsln] [--resume file.ckp] [--checkpoint file.ckp] [--mixed]\n"
+    "       hcfd_run --case case.ini [--dry-run]\n"
+    "The first form runs src/Test.cpp's case (dt = 1e-3; default ../wedge.hmsh, ../sols/MI355X,\n"
+    "1000 steps) through the MI355X drop-in classes; the second runs the case an INI file\n"
+    "describes ([mesh], [consts], [solver], [ICs], [BC-<name>], [time-integrator], [output]).\n";
+
 int main(int argc, char** argv) {
     try {
+        if (argc > 1 && (std::strcmp(argv[1], "--help") == 0 || std::strcmp(argv[1], "-h") == 0)) {
+            std::fputs(USAGE, stdout);
+            return 0;
+        }
         if (argc > 1 && std::strcmp(argv[1], "--case") == 0) {
             if (argc < 3) throw std::runtime_error("--case needs a file");
             bool dry = false;

@@ -485,6 +485,14 @@ def run_case(path, *extra):
     return subprocess.run([RUNNER, "--case", str(path)] + list(extra), capture_output=True, text=True, timeout=300)
 
 
+def test_hcfd_run_help():
+    """hcfd_run --help prints both command forms without touching a GPU or a file."""
+    import subprocess
+    r = subprocess.run([RUNNER, "--help"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    assert "hcfd_run --case case.ini" in r.stdout and "--checkpoint" in r.stdout
+
+
 def test_case_file_dry_run(tmp_path, wedge_path):
     """A case file resolves without a GPU: BCs by physical name onto BC ordinals as
     Euler::gather_BCs maps them (src/Base/Euler.cpp:42-60), the initial state at
