@@ -1,0 +1,115 @@
+"""Full-size run check of the four-quadrant Riemann problem on 4M triangles
+(2000x1000 quads split in two, nodes jittered +-20%, seed 0, cells shuffled) over
+its quoted 500 CFL steps at CFL 0.4: the GPU's default single-kernel step against
+the reference's own Base solver compiled from its sources (oracle/_ref,
+libhcfd_ref_nofma.so: src/Base/*.cpp and src/Mesh.cpp built with
+-ffp-contract=off), from the same initial state.
+
+The reference has no CFL step, so each of its steps takes the dt of the CFL
+formula (the oracle's orc_cfl_dt, pinned by its closed form) evaluated on the
+reference's own state; the GPU forms its dt on the device.  The states are
+compared bit for bit every 50 steps, and the clocks at the end of each segment.
+
+The reference takes about 4 s per step on one core of the GPU host, so the
+run is split into segments, each a process of its own, which can run side by
+side:
+
+    python tests/verify_c2_full_run.py [first last]     (default 0 500)
+
+A segment steps the GPU from the initial state to `last`, starts the reference
+from the GPU's state at step `first` (step 0: the common initial state; later:
+the state an earlier segment found equal to the reference's) with the GPU's
+clock there, and compares the states every 50 steps and at `last`, and the
+clocks at `last`.  Prints one progress line per 10 reference steps and exits 1
+on any difference.
+"""
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+import numpy as np  # noqa: E402
+
+import oracle_lib as ol  # noqa: E402
+from cfd_optimisations_amd import cases as C  # noqa: E402
+from cfd_optimisations_amd import lib as L  # noqa: E402
+from cfd_optimisations_amd.formats import write_hmsh  # noqa: E402
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def main(first: int, last: int) -> int:
+    if not ol.ref_available(ol.REF_NOFMA_SO):
+        say("the reference build oracle/_ref/libhcfd_ref_nofma.so is missing")
+        return 2
+    if not 0 <= first < last:
+        say("need 0 <= first < last")
+        return 2
+    w = C.WORKLOADS["c2_riemann_fp64"]
+    cfl, every = w["cfl"], 50
+    t0 = time.time()
+    case = C.riemann_case(w["nx"], w["ny"], seed=0)
+    hm = case.hmesh()
+    od = ol.OracleDomain(hm, case.gamma, case.bctypes, case.qbc)
+    od.zero_bc_iv()
+    U0 = od.initialise_prims(case.Qcells)
+    say(f"mesh: {hm.Nc:,} cells ({hm.Nc - (hm.Nf - hm.F_BC):,} interior), {hm.Nf:,} faces; "
+        f"{time.time() - t0:.1f} s; segment {first} -> {last}")
+
+    # the GPU run from the initial state, its state and clock kept at `first`
+    # and at every `every` steps after it
+    marks, clocks = {}, {0: 0.0}
+    with C.make_context(case, L.FP64, L.FLAG_ORDER_SFC) as ctx:
+        ctx.set_state(U0)
+        sched = ctx.schedule()
+        done = 0
+        stops = sorted({first, last} | set(range(first + every, last, every)))
+        for stop in stops:
+            if stop > done:
+                clocks[stop] = ctx.step(stop - done, cfl=cfl)
+                done = stop
+            marks[stop] = ctx.get_state() if stop else U0
+        fault = ctx.fault()
+    say(f"GPU: {last} steps, schedule {sched}, t = {clocks[last]!r}, fault flag {fault}")
+
+    # the reference from the GPU's state at `first`, one step at a time with the
+    # CFL dt of its own state
+    bymap = np.zeros(int(hm.coltag.max()) + 1, np.int32)
+    for b in range(hm.NBCcol):
+        bymap[hm.coltag[hm.Ncol - hm.NBCcol + b]] = case.bctypes[b]
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "c2.hmsh")
+        write_hmsh(path, hm)
+        rc = ol.RefCase(path, case.gamma, bymap, case.qbc, ol.REF_NOFMA_SO)
+    rc.zero_bc_iv()
+    U, t_ref = marks[first], clocks[first]
+    t1 = time.time()
+    ok = True
+    for s in range(first + 1, last + 1):
+        dt = od.cfl_dt(U, cfl)
+        U = rc.run(U, 1, dt)
+        t_ref += dt
+        if s % 10 == 0:
+            say(f"reference step {s}/{last}, t = {t_ref:.6e}, {(time.time() - t1) / (s - first):.2f} s per step")
+        if s in marks:
+            same = np.array_equal(U, marks[s])
+            diff = np.abs(U - marks[s]).max() / np.abs(U).max()
+            say(f"step {s}: GPU {'==' if same else '!='} reference (max |dU| / max |U| = {diff:.3e})")
+            ok = ok and same
+    rc.close()
+    same_t = t_ref == clocks[last]
+    say(f"clock at step {last}: GPU {clocks[last]!r}, reference {t_ref!r} ({'equal' if same_t else 'different'})")
+    ok = ok and same_t and fault == 0
+    say(f"PASS: steps {first} -> {last} bit-identical at every step checked" if ok else "FAIL")
+    return 0 if ok else 1
+
+
+if __name__ == "__main__":
+    a = [int(x) for x in sys.argv[1:3]]
+    sys.exit(main(*(a if len(a) == 2 else (0, C.WORKLOADS["c2_riemann_fp64"]["steps"]))))
