@@ -1,6 +1,8 @@
 """Per-instance agreement of the 256-wedge FP64 batch (Mach U[1.5, 4.0], angle
 U[5, 25] deg, 20k-500k triangles log-uniform, seed 1) with the reference's own
-CPU solver over the quoted 2,000 CFL steps at CFL 0.5.
+CPU solver over the quoted 2,000 CFL steps at CFL 0.5 (--batch c5: the same for
+the 1,024 wedges of the mixed-precision sweep run in FP64, the comparator of its
+error figures).
 
     python tests/verify_c4_instances.py [--count N --max-triangles T | --range A B]
                                         [--jobs J] [--log-dir D]
@@ -18,7 +20,7 @@ Base solver compiled from its sources with -ffp-contract=off
 (oracle/_ref/libhcfd_ref_nofma.so) on the instance's mesh from the same initial
 state, each step with the CFL dt of its own state (the oracle's orc_cfl_dt: the
 reference has no CFL step), compared bit for bit with the GPU's state and clock.
-Each child writes its progress to <log_dir>/c4_ref_<instance>.log.
+Each child writes its progress to <log_dir>/<batch>_ref_<instance>.log.
 
 A script rather than a collected test: the reference needs several minutes of
 one CPU core per instance.  Exits 1 if any instance differs.
@@ -91,11 +93,14 @@ def main(argv=None) -> int:
     ap.add_argument("--range", type=int, nargs=2)
     ap.add_argument("--jobs", type=int, default=16)
     ap.add_argument("--log-dir", default=os.path.join(ROOT, "gpurun_out"))
+    ap.add_argument("--batch", choices=("c4", "c5"), default="c4",
+                    help="c5: the 1,024 wedges of the mixed-precision sweep, run here in FP64 (the "
+                         "comparator of its error figures); instances that fault are skipped")
     a = ap.parse_args(argv)
     if not ol.ref_available(ol.REF_NOFMA_SO):
         say("the reference build oracle/_ref/libhcfd_ref_nofma.so is missing")
         return 2
-    w = C.WORKLOADS["c4_batch_fp64"]
+    w = C.WORKLOADS["c4_batch_fp64" if a.batch == "c4" else "c5_batch_mixed"]
     steps, cfl = w["steps"], w["cfl"]
     specs = C.wedge_batch_specs(w["n"], w["seed"], w["mach"], w["angle"], w["tris"])
     cases = [C.ramp_case(sp["nx"], sp["ny"], sp["mach"], sp["angle"], seed=sp["seed"]) for sp in specs]
@@ -120,6 +125,10 @@ def main(argv=None) -> int:
             sched = ctx.schedule()
         say(f"GPU: {steps} steps of the batch, schedule {sched}, {int(fb.sum())} faulted instances, "
             f"{time.time() - t0:.1f} s")
+        skipped = [k for k in pick if fb[k]]
+        if skipped:
+            say(f"skipped (faulted on the GPU, as under the reference method): {skipped}")
+            pick = [k for k in pick if not fb[k]]
         jobs = []
         for k in sorted(pick, key=lambda k: -tris[k]):
             mesh = os.path.join(td, f"w{k}.hmsh")
@@ -127,7 +136,7 @@ def main(argv=None) -> int:
             npz = os.path.join(td, f"w{k}.npz")
             np.savez(npz, inst=k, steps=steps, cfl=cfl, gamma=C.GAMMA, mesh=mesh, bctypes=cases[k].bctypes,
                      qbc=cases[k].qbc, U0=U0[k], U=Ug[k], t=tb[k], mach=specs[k]["mach"], angle=specs[k]["angle"])
-            jobs.append((k, npz, os.path.join(a.log_dir, f"c4_ref_{k}.log")))
+            jobs.append((k, npz, os.path.join(a.log_dir, f"{a.batch}_ref_{k}.log")))
         del U0, Ug
         running, bad, done = [], 0, 0
         while jobs or running:
