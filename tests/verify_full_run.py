@@ -1,20 +1,24 @@
-"""Full-size run check of the four-quadrant Riemann problem on 4M triangles
-(2000x1000 quads split in two, nodes jittered +-20%, seed 0, cells shuffled) over
-its quoted 500 CFL steps at CFL 0.4: the GPU's default single-kernel step against
-the reference's own Base solver compiled from its sources (oracle/_ref,
+"""Full-size run check of the quoted single-mesh workloads in FP64 against the
+reference's own Base solver compiled from its sources (oracle/_ref,
 libhcfd_ref_nofma.so: src/Base/*.cpp and src/Mesh.cpp built with
--ffp-contract=off), from the same initial state.
+-ffp-contract=off), from the same initial state:
+  c2: the four-quadrant Riemann problem on 4M triangles (2000x1000 quads split
+      in two, nodes jittered +-20%, seed 0, cells shuffled), 500 CFL steps at
+      CFL 0.4;
+  c3: Mach 3 over a 20 deg ramp on 16M triangles (4000x2000 quads), 1,000 CFL
+      steps at CFL 0.5.
+The GPU runs its default single-kernel step.
 
 The reference has no CFL step, so each of its steps takes the dt of the CFL
 formula (the oracle's orc_cfl_dt, pinned by its closed form) evaluated on the
 reference's own state; the GPU forms its dt on the device.  The states are
 compared bit for bit every 50 steps, and the clocks at the end of each segment.
 
-The reference takes about 4 s per step on one core of the GPU host, so the
-run is split into segments, each a process of its own, which can run side by
-side:
+The reference takes about 4 s (c2) or 17 s (c3) per step on one core of the GPU
+host, so a run is split into segments, each a process of its own, which can run
+side by side:
 
-    python tests/verify_c2_full_run.py [first last]     (default 0 500)
+    python tests/verify_full_run.py [--workload c2|c3] [first last]
 
 A segment steps the GPU from the initial state to `last`, starts the reference
 from the GPU's state at step `first` (step 0: the common initial state; later:
@@ -44,21 +48,25 @@ def say(msg):
     print(msg, flush=True)
 
 
-def main(first: int, last: int) -> int:
+def main(workload: str, first: int, last: int) -> int:
     if not ol.ref_available(ol.REF_NOFMA_SO):
         say("the reference build oracle/_ref/libhcfd_ref_nofma.so is missing")
         return 2
     if not 0 <= first < last:
         say("need 0 <= first < last")
         return 2
-    w = C.WORKLOADS["c2_riemann_fp64"]
-    cfl, every = w["cfl"], 50
     t0 = time.time()
-    case = C.riemann_case(w["nx"], w["ny"], seed=0)
+    if workload == "c2":
+        w = C.WORKLOADS["c2_riemann_fp64"]
+        case = C.riemann_case(w["nx"], w["ny"], seed=0)
+    else:
+        w = C.WORKLOADS["c3_ramp_16m"]
+        case = C.ramp_case(w["nx"], w["ny"], w["mach"], w["angle"], seed=0)
+    cfl, every = w["cfl"], 50
     hm = case.hmesh()
     od = ol.OracleDomain(hm, case.gamma, case.bctypes, case.qbc)
     od.zero_bc_iv()
-    U0 = od.initialise_prims(case.Qcells)
+    U0 = od.initialise_prims(case.Qcells) if case.Qcells is not None else od.initialise(case.qic)
     say(f"mesh: {hm.Nc:,} cells ({hm.Nc - (hm.Nf - hm.F_BC):,} interior), {hm.Nf:,} faces; "
         f"{time.time() - t0:.1f} s; segment {first} -> {last}")
 
@@ -84,7 +92,7 @@ def main(first: int, last: int) -> int:
     for b in range(hm.NBCcol):
         bymap[hm.coltag[hm.Ncol - hm.NBCcol + b]] = case.bctypes[b]
     with tempfile.TemporaryDirectory() as td:
-        path = os.path.join(td, "c2.hmsh")
+        path = os.path.join(td, "mesh.hmsh")
         write_hmsh(path, hm)
         rc = ol.RefCase(path, case.gamma, bymap, case.qbc, ol.REF_NOFMA_SO)
     rc.zero_bc_iv()
@@ -111,5 +119,12 @@ def main(first: int, last: int) -> int:
 
 
 if __name__ == "__main__":
-    a = [int(x) for x in sys.argv[1:3]]
-    sys.exit(main(*(a if len(a) == 2 else (0, C.WORKLOADS["c2_riemann_fp64"]["steps"]))))
+    args = sys.argv[1:]
+    wl = "c2"
+    if len(args) >= 2 and args[0] == "--workload":
+        wl, args = args[1], args[2:]
+    if wl not in ("c2", "c3"):
+        sys.exit("--workload is c2 or c3")
+    quoted = C.WORKLOADS["c2_riemann_fp64" if wl == "c2" else "c3_ramp_16m"]["steps"]
+    a = [int(x) for x in args[:2]]
+    sys.exit(main(wl, *(a if len(a) == 2 else (0, quoted))))
