@@ -10,7 +10,7 @@ error figures).
 Phase 1 (this process, the GPU): the whole batch in one context (default
 schedule, hipGraph replay), 2,000 steps.  For the instances checked (--range:
 instances A..B-1; otherwise --count instances, default 16, chosen evenly over
-those of at most --max-triangles triangles, default 200,000) it writes the
+those of --min-triangles to --max-triangles triangles, default 0 to 200,000) it writes the
 instance's mesh as .hmsh and its initial state, final state and clock to a
 temporary directory.
 
@@ -90,6 +90,7 @@ def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--count", type=int, default=16)
     ap.add_argument("--max-triangles", type=int, default=200000)
+    ap.add_argument("--min-triangles", type=int, default=0)
     ap.add_argument("--range", type=int, nargs=2)
     ap.add_argument("--jobs", type=int, default=16)
     ap.add_argument("--log-dir", default=os.path.join(ROOT, "gpurun_out"))
@@ -109,11 +110,11 @@ def main(argv=None) -> int:
         pick = list(range(max(a.range[0], 0), min(a.range[1], len(cases))))
         say(f"{len(cases)} instances generated; checking instances {pick[0]}..{pick[-1]}")
     else:
-        small = [k for k in np.argsort(tris, kind="stable") if tris[k] <= a.max_triangles]
+        small = [k for k in np.argsort(tris, kind="stable") if a.min_triangles <= tris[k] <= a.max_triangles]
         n = a.count
         pick = sorted({int(small[int(round(j * (len(small) - 1) / max(n - 1, 1)))]) for j in range(n)})
-        say(f"{len(cases)} instances generated; checking {len(pick)} of the {len(small)} with at most "
-            f"{a.max_triangles:,} triangles: {pick}")
+        say(f"{len(cases)} instances generated; checking {len(pick)} of the {len(small)} with "
+            f"{a.min_triangles:,} to {a.max_triangles:,} triangles: {pick}")
     os.makedirs(a.log_dir, exist_ok=True)
     with tempfile.TemporaryDirectory() as td:
         t0 = time.time()
