@@ -6,7 +6,9 @@ libhcfd_ref_nofma.so: src/Base/*.cpp and src/Mesh.cpp built with
       in two, nodes jittered +-20%, seed 0, cells shuffled), 500 CFL steps at
       CFL 0.4;
   c3: Mach 3 over a 20 deg ramp on 16M triangles (4000x2000 quads), 1,000 CFL
-      steps at CFL 0.5.
+      steps at CFL 0.5;
+  c1: the supersonic 15 deg wedge (Mach 2.5, 224x112 quads, about 50k
+      triangles), 2,000 CFL steps at CFL 0.5.
 The GPU runs its default single-kernel step.
 
 The reference has no CFL step, so each of its steps takes the dt of the CFL
@@ -18,7 +20,7 @@ The reference takes about 4 s (c2) or 17 s (c3) per step on one core of the GPU
 host, so a run is split into segments, each a process of its own, which can run
 side by side:
 
-    python tests/verify_full_run.py [--workload c2|c3] [first last]
+    python tests/verify_full_run.py [--workload c1|c2|c3] [first last]
 
 A segment steps the GPU from the initial state to `last`, starts the reference
 from the GPU's state at step `first` (step 0: the common initial state; later:
@@ -59,6 +61,9 @@ def main(workload: str, first: int, last: int) -> int:
     if workload == "c2":
         w = C.WORKLOADS["c2_riemann_fp64"]
         case = C.riemann_case(w["nx"], w["ny"], seed=0)
+    elif workload == "c1":
+        w = C.WORKLOADS["c1_wedge_cpu"]
+        case = C.ramp_case(w["nx"], w["ny"], w["mach"], w["angle"], seed=0)
     else:
         w = C.WORKLOADS["c3_ramp_16m"]
         case = C.ramp_case(w["nx"], w["ny"], w["mach"], w["angle"], seed=0)
@@ -123,8 +128,8 @@ if __name__ == "__main__":
     wl = "c2"
     if len(args) >= 2 and args[0] == "--workload":
         wl, args = args[1], args[2:]
-    if wl not in ("c2", "c3"):
-        sys.exit("--workload is c2 or c3")
-    quoted = C.WORKLOADS["c2_riemann_fp64" if wl == "c2" else "c3_ramp_16m"]["steps"]
+    if wl not in ("c1", "c2", "c3"):
+        sys.exit("--workload is c1, c2 or c3")
+    quoted = C.WORKLOADS[{"c1": "c1_wedge_cpu", "c2": "c2_riemann_fp64", "c3": "c3_ramp_16m"}[wl]]["steps"]
     a = [int(x) for x in args[:2]]
     sys.exit(main(wl, *(a if len(a) == 2 else (0, quoted))))
